@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the PT iteration on one MI355X.
+
+A "step" is one PT iteration (the single-pass kernel in 2D: Pt/τ update,
+V update and free-slip copies for the whole grid).  The headline workload is
+2D Maxwell visco-elastic Stokes at nx = ny = 8191 in FP64 (dt = η/G/10),
+reported as the reference's effective throughput T_eff = A_eff / t_it with
+A_eff = 18·nx·ny·8 B (scripts/Stokes2D_ve_perf.jl:173: 6 fields read and
+written + 6 read-only).  The 3D VE 511³ block (A_eff = 29·n³·8 B,
+Stokes3D_ve_perf.jl:229) is reported alongside.
+
+Timing: HIP events recorded on the library's own stream around exactly
+`--steps` iterations, after `--warmup` untimed ones, with the stream drained
+on both sides.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pseudotransientstokes.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+A100_2D_VE_8191_GBS = 932.8  # output/out_Stokes2D_ve_perf_ampere.txt:45
+METRIC = "T_eff GB/s (% of HBM peak) + time-to-‖res‖∞<1e-8, FP32-refined vs FP64"
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (the one libpts launches on)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        rc = self.hip.hipEventCreate(ctypes.byref(e))
+        if rc != 0:
+            raise RuntimeError(f"hipEventCreate failed ({rc})")
+        return e
+
+    def record(self, e, stream):
+        if self.hip.hipEventRecord(e, ctypes.c_void_p(stream)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed_ms(self, a, b):
+        self.hip.hipEventSynchronize(b)
+        ms = ctypes.c_float()
+        if self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+
+def time_iterations(model, steps, warmup, ev):
+    """Wall and event time of `steps` PT iterations after `warmup` ones."""
+    model.iterate(warmup)
+    model.ctx.sync()
+    s = model.ctx.stream()
+    a, b = ev.event(), ev.event()
+    t0 = time.perf_counter()
+    ev.record(a, s)
+    model.iterate(steps)
+    ev.record(b, s)
+    model.ctx.sync()
+    wall = time.perf_counter() - t0
+    return ev.elapsed_ms(a, b), wall
+
+
+def bench_2d(n, steps, warmup, ev):
+    from ptstokes import StokesModel
+    # 2D Maxwell VE: G = η = 1, dt = η/G/10, pure shear εbg = 1, 1e-3 inclusion of radius 1
+    m = StokesModel("Stokes2D_ve_perf", n=n, dt=0.1).setup()
+    m.ctx.assign_tau()
+    ms, wall = time_iterations(m, steps, warmup, ev)
+    t_it = ms / 1e3 / steps
+    a_eff = 18.0 * n * n * 8
+    out = dict(n=n, steps=steps, ms_per_it=ms / steps, wall_ms_per_it=wall * 1e3 / steps, A_eff_bytes=a_eff,
+               T_eff_GBs=a_eff / t_it / 1e9, device_bytes=m.ctx.device_bytes())
+    m.close()
+    return out
+
+
+def bench_3d(n, steps, warmup, ev):
+    from ptstokes import StokesModel
+    m = StokesModel("Stokes3D_ve_perf", n=n).setup()
+    m.ctx.assign_tau()
+    ms, wall = time_iterations(m, steps, warmup, ev)
+    t_it = ms / 1e3 / steps
+    a_eff = 29.0 * n ** 3 * 8
+    out = dict(n=n, steps=steps, ms_per_it=ms / steps, wall_ms_per_it=wall * 1e3 / steps, A_eff_bytes=a_eff,
+               T_eff_GBs=a_eff / t_it / 1e9, device_bytes=m.ctx.device_bytes())
+    m.close()
+    return out
+
+
+def cpu_baseline(n, iters, threads):
+    """The oracle (the reference's CPU kernels restated in C, OpenMP over the
+    stencil loops) timed on the host cores at the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    o = Oracle("Stokes2D_ve_perf", n=n, dt=0.1, threads=threads).setup()
+    o.assign_tau()
+    o.iterate(1)  # first touch
+    t0 = time.perf_counter()
+    o.iterate(iters)
+    t = (time.perf_counter() - t0) / iters
+    a_eff = 18.0 * n * n * 8
+    return dict(value=a_eff / t / 1e9, unit="GB/s", cores=threads, kind="port",
+                sample=f"{iters} PT iterations of 2D VE {n}x{n} FP64 (perf-script kernels, "
+                       f"{threads} OpenMP threads) after setup; ms/it {t * 1e3:.1f}")
+
+
+def load_traffic(key):
+    p = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n2d", type=int, default=8191)
+    ap.add_argument("--n3d", type=int, default=511)
+    ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU path is a design, not a build, in this repository)", file=sys.stderr)
+        return 2
+
+    ev = HipEvents()
+    r2 = bench_2d(args.n2d, args.steps, args.warmup, ev)
+    print(f"# 2D VE {args.n2d}^2: {r2['ms_per_it']:.4f} ms/it, T_eff {r2['T_eff_GBs']:.1f} GB/s", file=sys.stderr,
+          flush=True)
+    extra = {}
+    if not args.no_3d:
+        s3 = max(20, args.steps // 5)
+        r3 = bench_3d(args.n3d, s3, max(5, args.warmup // 2), ev)
+        r3["frac_of_peak"] = r3["T_eff_GBs"] / HBM_PEAK_GBS
+        r3["workload"] = f"3D Maxwell VE Stokes, nx=ny=nz={args.n3d}, FP64, x-z pure shear"
+        extra["3d_ve"] = r3
+        print(f"# 3D VE {args.n3d}^3: {r3['ms_per_it']:.4f} ms/it, T_eff {r3['T_eff_GBs']:.1f} GB/s",
+              file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args.n2d, args.cpu_iters, threads)
+        print(f"# CPU baseline: {cpu['value']:.2f} GB/s ({cpu['sample']})", file=sys.stderr, flush=True)
+
+    t_eff = r2["T_eff_GBs"]
+    line = {
+        "metric": METRIC,
+        "value": round(t_eff, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r2["ms_per_it"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(t_eff / A100_2D_VE_8191_GBS, 3),
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2D Maxwell visco-elastic Stokes PT iteration, nx=ny={args.n2d}, FP64, "
+                        "G=eta=1, dt=0.1, pure shear, 1e-3 inclusion r=1 (single-pass kernel)",
+            "A_eff_bytes_per_step": r2["A_eff_bytes"],
+            "T_eff_frac_of_hbm_peak": round(t_eff / HBM_PEAK_GBS, 4),
+            "baseline": "A100 T_eff 932.8 GB/s, output/out_Stokes2D_ve_perf_ampere.txt:45",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(t_eff, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(t_eff / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic(f"k2_iter_{args.n2d}"),
+        },
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

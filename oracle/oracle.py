@@ -97,6 +97,8 @@ def load():
         lib.or_solve.restype = ctypes.c_int
         lib.or_solve.argtypes = [ctypes.c_void_p, P(ctypes.c_int64), P(ctypes.c_double), P(ctypes.c_int64)]
         lib.or_scalars.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
+        lib.or_kernel.restype = ctypes.c_int
+        lib.or_kernel.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         _lib = lib
     return _lib
 
@@ -162,6 +164,16 @@ class Oracle:
 
     def compute_res(self):
         self.lib.or_compute_res(self.h)
+
+    KERNELS = {"compute_P": 0, "compute_tau": 1, "compute_dV": 2, "compute_V": 3, "compute_Res": 4,
+               "compute_Pt_tau": 5, "compute_V_perf": 6, "compute_Res_perf": 7, "bcs_V": 8,
+               "bc_x": 10, "bc_y": 11, "bc_z": 12}
+
+    def kernel(self, name, field=None):
+        arg = FIELD_ID[field] if field is not None else 0
+        rc = self.lib.or_kernel(self.h, self.KERNELS[name], arg)
+        if rc != 0:
+            raise ValueError(name)
 
     def check(self):
         out = (ctypes.c_double * 9)()

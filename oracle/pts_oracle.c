@@ -849,6 +849,25 @@ void or_iterate(or_state* s, int64_t n) {
     }
 }
 
+int or_kernel(or_state* s, int which, int arg) {
+    const int d2 = s->p.ndim == 2;
+    switch (which) {
+        case 0: if (d2) p2_compute_P(s); else p3_compute_P(s); return 0;
+        case 1: if (d2) p2_compute_tau(s); else p3_compute_tau(s); return 0;
+        case 2: if (d2) p2_compute_dV(s, 1); else p3_compute_dV(s); return 0;
+        case 3: if (d2) p2_compute_V(s); else p3_compute_V(s); return 0;
+        case 4: if (d2) p2_compute_Res(s); else p3_compute_Res(s); return 0;
+        case 5: if (d2) f2_compute_Pt_tau(s); else f3_compute_Pt_tau(s); return 0;
+        case 6: if (d2) f2_compute_V(s); else f3_compute_V(s); return 0;
+        case 7: if (d2) f2_compute_Res(s); else f3_compute_Res(s); return 0;
+        case 8: bcs_V(s); return 0;
+        case 10: if (arg < 0 || arg >= ORF_NFIELDS) return -1; bc_x(s->f[arg]); return 0;
+        case 11: if (arg < 0 || arg >= ORF_NFIELDS) return -1; bc_y(s->f[arg]); return 0;
+        case 12: if (arg < 0 || arg >= ORF_NFIELDS) return -1; bc_z(s->f[arg]); return 0;
+        default: return -1;
+    }
+}
+
 void or_compute_res(or_state* s) {
     if (s->p.ndim == 2) {
         if (s->p.perf) f2_compute_Res(s);

@@ -74,6 +74,12 @@ void      or_check(or_state* s, double out[9]);
 /* Full time loop (for it = 1:nt ...).  Returns 0, or -1 on NaN (the reference's
  * error("NaN")).  ittot/err/iters-per-step written to the out pointers. */
 int       or_solve(or_state* s, int64_t* ittot, double* err_last, int64_t* iters_per_step);
+/* Run one reference kernel in isolation (per-kernel parity checks):
+ * 0 compute_P!, 1 compute_τ!, 2 compute_dV!, 3 compute_V!, 4 compute_Res! (divide form);
+ * 5 compute_Pt_τ!, 6 compute_V! (perf), 7 compute_Res! (perf);
+ * 8 the script's free-slip calls on the velocities;
+ * 10 bc_x!, 11 bc_y!, 12 bc_z! on field `arg`. Returns -1 for an unknown id. */
+int       or_kernel(or_state* s, int which, int arg);
 /* Scalars derived by the script: dx,dy,dz,_dx,_dy,_dz,max_l,Vpdτ. */
 void      or_scalars(const or_state* s, double out[8]);
 

@@ -1,0 +1,714 @@
+// pts_api.hip -- the C ABI (include/pts.h): context, device memory, host
+// copies, per-kernel entry points and the device-resident PT loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+
+#include "pts_internal.hpp"
+
+using namespace pts;
+
+namespace {
+
+// Fields that pts_pt_iterate advances with ping-pong buffers.
+constexpr int kState[] = {PTS_PT, PTS_VX, PTS_VY, PTS_VZ, PTS_TXX, PTS_TYY, PTS_TZZ, PTS_TXY, PTS_TXZ, PTS_TYZ};
+constexpr int kVel[] = {PTS_VX, PTS_VY, PTS_VZ};
+
+std::string g_create_error;
+
+}  // namespace
+
+struct pts_ctx {
+    pts_grid g{};
+    int nd = 2, nx = 0, ny = 0, nz = 1;
+    int sx = 0, sxy = 0;
+    int64_t nalloc = 0;  // elements per field allocation
+    double* cur[PTS_NFIELDS] = {};
+    double* alt[PTS_NFIELDS] = {};
+    bool alt_synced = false;
+    hipStream_t stream = nullptr;
+    double* red_partials = nullptr;
+    double* red_out = nullptr;
+    double* h_red = nullptr;
+    int64_t bytes = 0;
+    int rows_per_block = 128;
+    std::string err;
+    // graphs of n single-pass iterations keyed by (n, parity of the start buffer)
+    std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
+    int parity = 0;  // flips each single-pass iteration (which physical buffer is `cur`)
+};
+
+namespace {
+
+int fail(pts_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                             \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return fail((ctx), PTS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+bool field_valid(const pts_ctx* c, int f) {
+    if (f < 0 || f >= PTS_NFIELDS) return false;
+    if (c->nd == 2) {
+        switch (f) {
+            case PTS_VZ: case PTS_TZZ: case PTS_TXZ: case PTS_TYZ: case PTS_TZZ_O: case PTS_TXZ_O:
+            case PTS_TYZ_O: case PTS_RZ: case PTS_DVZ:
+                return false;
+            default:
+                return true;
+        }
+    }
+    return true;
+}
+
+int ensure(pts_ctx* c, int f) {
+    if (!field_valid(c, f)) return fail(c, PTS_ERR_STATE, "field " + std::to_string(f) + " not valid for this grid");
+    if (c->cur[f]) return PTS_OK;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)c->nalloc * sizeof(double));
+    if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    HIPCHK(c, hipMemsetAsync(p, 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    c->cur[f] = (double*)p;
+    c->bytes += c->nalloc * (int64_t)sizeof(double);
+    return PTS_OK;
+}
+
+int ensure_list(pts_ctx* c, std::initializer_list<int> fs) {
+    for (int f : fs) {
+        int rc = ensure(c, f);
+        if (rc) return rc;
+    }
+    return PTS_OK;
+}
+
+int ensure_alt(pts_ctx* c, int f) {
+    if (c->alt[f]) return PTS_OK;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)c->nalloc * sizeof(double));
+    if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    HIPCHK(c, hipMemsetAsync(p, 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    c->alt[f] = (double*)p;
+    c->bytes += c->nalloc * (int64_t)sizeof(double);
+    c->alt_synced = false;
+    return PTS_OK;
+}
+
+Fields fields_of(const pts_ctx* c, double* const* src) {
+    Fields f;
+    for (int q = 0; q < PTS_NFIELDS; ++q) f.p[q] = src[q];
+    f.sx = c->sx;
+    f.sxy = c->sxy;
+    f.nx = c->nx;
+    f.ny = c->ny;
+    f.nz = c->nz;
+    return f;
+}
+Fields F(const pts_ctx* c) { return fields_of(c, c->cur); }
+
+// Any in-place change of the velocities invalidates the Dirichlet copy
+// held by the ping-pong partner buffers.
+void touched(pts_ctx* c) { c->alt_synced = false; }
+
+int setdev(pts_ctx* c) {
+    HIPCHK(c, hipSetDevice(c->g.device));
+    return PTS_OK;
+}
+
+StepScalars scal(double r, double G, double dt, double dx, double dy, double dz) {
+    StepScalars s;
+    s.r = r;
+    s.Gdt = G * dt;
+    s.dx = dx; s.dy = dy; s.dz = dz;
+    s._dx = 1.0 / dx; s._dy = 1.0 / dy; s._dz = 1.0 / dz;
+    return s;
+}
+
+int state_fields(const pts_ctx* c, int* out) {
+    int n = 0;
+    for (int f : kState)
+        if (field_valid(c, f)) out[n++] = f;
+    return n;
+}
+
+}  // namespace
+
+namespace pts {
+void field_shape(int ndim, int nx, int ny, int nz, int f, int64_t s[3]) {
+    s[0] = nx; s[1] = ny; s[2] = ndim == 3 ? nz : 1;
+    if (ndim == 2) {
+        switch (f) {
+            case PTS_VX: s[0] = nx + 1; break;
+            case PTS_VY: s[1] = ny + 1; break;
+            case PTS_TXY: case PTS_TXY_O: s[0] = nx - 1; s[1] = ny - 1; break;
+            case PTS_RX: case PTS_DVX: s[0] = nx - 1; s[1] = ny - 2; break;
+            case PTS_RY: case PTS_DVY: s[0] = nx - 2; s[1] = ny - 1; break;
+            default: break;
+        }
+        return;
+    }
+    switch (f) {
+        case PTS_VX: s[0] = nx + 1; break;
+        case PTS_VY: s[1] = ny + 1; break;
+        case PTS_VZ: s[2] = nz + 1; break;
+        case PTS_TXX: case PTS_TXX_O: s[1] = ny - 2; s[2] = nz - 2; break;
+        case PTS_TYY: case PTS_TYY_O: s[0] = nx - 2; s[2] = nz - 2; break;
+        case PTS_TZZ: case PTS_TZZ_O: s[0] = nx - 2; s[1] = ny - 2; break;
+        case PTS_TXY: case PTS_TXY_O: s[0] = nx - 1; s[1] = ny - 1; s[2] = nz - 2; break;
+        case PTS_TXZ: case PTS_TXZ_O: s[0] = nx - 1; s[1] = ny - 2; s[2] = nz - 1; break;
+        case PTS_TYZ: case PTS_TYZ_O: s[0] = nx - 2; s[1] = ny - 1; s[2] = nz - 1; break;
+        case PTS_RX: case PTS_DVX: s[0] = nx - 1; s[1] = ny - 2; s[2] = nz - 2; break;
+        case PTS_RY: case PTS_DVY: s[0] = nx - 2; s[1] = ny - 1; s[2] = nz - 2; break;
+        case PTS_RZ: case PTS_DVZ: s[0] = nx - 2; s[1] = ny - 2; s[2] = nz - 1; break;
+        default: break;
+    }
+}
+}  // namespace pts
+
+extern "C" {
+
+int pts_create(const pts_grid* g, pts_ctx** out) {
+    if (!g || !out) return fail(nullptr, PTS_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (g->ndim != 2 && g->ndim != 3) return fail(nullptr, PTS_ERR_ARG, "ndim must be 2 or 3");
+    const int nz = g->ndim == 3 ? g->nz : 1;
+    if (g->nx < 3 || g->ny < 3 || (g->ndim == 3 && g->nz < 3))
+        return fail(nullptr, PTS_ERR_ARG, "each of nx, ny (, nz) must be >= 3");
+    if (g->nx > 65000 || g->ny > 65000 || nz > 65000) return fail(nullptr, PTS_ERR_ARG, "grid dimension too large");
+    if (g->fd_mode != PTS_FD_DIV && g->fd_mode != PTS_FD_MUL) return fail(nullptr, PTS_ERR_ARG, "bad fd_mode");
+    pts_ctx* c = new pts_ctx();
+    c->g = *g;
+    c->nd = g->ndim;
+    c->nx = g->nx;
+    c->ny = g->ny;
+    c->nz = nz;
+    c->sx = ((g->nx + 1) + 31) / 32 * 32;
+    const int64_t sxy = (int64_t)c->sx * (g->ny + 1);
+    const int64_t na = sxy * (g->ndim == 3 ? nz + 1 : 1);
+    if (na >= (int64_t)1 << 31) {
+        delete c;
+        return fail(nullptr, PTS_ERR_ARG, "grid too large for 32-bit element offsets");
+    }
+    c->sxy = (int)sxy;
+    c->nalloc = na;
+    if (const char* e = std::getenv("PTS_ROWS_PER_BLOCK")) c->rows_per_block = std::max(1, std::atoi(e));
+    hipError_t e = hipSetDevice(g->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->red_partials, sizeof(double) * kRedBlocks * kRedVals);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->red_out, sizeof(double) * kRedVals);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_red, sizeof(double) * kRedVals, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        std::string m = std::string("HIP init: ") + hipGetErrorString(e);
+        pts_destroy(c);
+        return fail(nullptr, PTS_ERR_HIP, m);
+    }
+    *out = c;
+    return PTS_OK;
+}
+
+int pts_destroy(pts_ctx* c) {
+    if (!c) return PTS_OK;
+    (void)hipSetDevice(c->g.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (int q = 0; q < PTS_NFIELDS; ++q) {
+        if (c->cur[q]) (void)hipFree(c->cur[q]);
+        if (c->alt[q]) (void)hipFree(c->alt[q]);
+    }
+    if (c->red_partials) (void)hipFree(c->red_partials);
+    if (c->red_out) (void)hipFree(c->red_out);
+    if (c->h_red) (void)hipHostFree(c->h_red);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PTS_OK;
+}
+
+const char* pts_last_error(const pts_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int pts_field_shape(const pts_ctx* c, int f, int64_t shape[3]) {
+    if (!c || !shape) return PTS_ERR_ARG;
+    if (!field_valid(c, f)) return fail(const_cast<pts_ctx*>(c), PTS_ERR_STATE, "field not valid for this grid");
+    field_shape(c->nd, c->nx, c->ny, c->nz, f, shape);
+    return PTS_OK;
+}
+
+int pts_get_stream(const pts_ctx* c, void** stream) {
+    if (!c || !stream) return PTS_ERR_ARG;
+    *stream = (void*)c->stream;
+    return PTS_OK;
+}
+
+int pts_device_bytes(const pts_ctx* c, int64_t* bytes) {
+    if (!c || !bytes) return PTS_ERR_ARG;
+    *bytes = c->bytes;
+    return PTS_OK;
+}
+
+static int copy_field(pts_ctx* c, int f, double* host, const double* chost, int64_t count, bool up) {
+    if (!c || (up ? !chost : !host)) return fail(c, PTS_ERR_ARG, "null argument");
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure(c, f)) return rc;
+    int64_t s[3];
+    field_shape(c->nd, c->nx, c->ny, c->nz, f, s);
+    if (count != s[0] * s[1] * s[2])
+        return fail(c, PTS_ERR_ARG, "count " + std::to_string(count) + " != field size " + std::to_string(s[0] * s[1] * s[2]));
+    if (count == 0) return PTS_OK;
+    const size_t w = (size_t)s[0] * sizeof(double);
+    for (int64_t k = 0; k < s[2]; ++k) {
+        double* dev = c->cur[f] + k * (int64_t)c->sxy;
+        if (up)
+            HIPCHK(c, hipMemcpy2DAsync(dev, (size_t)c->sx * sizeof(double), chost + k * s[0] * s[1], w, w, (size_t)s[1],
+                                       hipMemcpyHostToDevice, c->stream));
+        else
+            HIPCHK(c, hipMemcpy2DAsync(host + k * s[0] * s[1], w, dev, (size_t)c->sx * sizeof(double), w, (size_t)s[1],
+                                       hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (up) touched(c);
+    return PTS_OK;
+}
+
+int pts_upload(pts_ctx* c, int f, const double* host, int64_t count) { return copy_field(c, f, nullptr, host, count, true); }
+int pts_download(pts_ctx* c, int f, double* host, int64_t count) { return copy_field(c, f, host, nullptr, count, false); }
+
+int pts_fill(pts_ctx* c, int f, double v) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure(c, f)) return rc;
+    int64_t s[3];
+    field_shape(c->nd, c->nx, c->ny, c->nz, f, s);
+    HIPCHK(c, launch_fill(c->stream, c->cur[f], v, F(c), s));
+    touched(c);
+    return PTS_OK;
+}
+
+int pts_copy(pts_ctx* c, int dst, int src) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure_list(c, {dst, src})) return rc;
+    int64_t a[3], b[3];
+    field_shape(c->nd, c->nx, c->ny, c->nz, dst, a);
+    field_shape(c->nd, c->nx, c->ny, c->nz, src, b);
+    if (a[0] != b[0] || a[1] != b[1] || a[2] != b[2]) return fail(c, PTS_ERR_ARG, "copy: shape mismatch");
+    HIPCHK(c, launch_copy(c->stream, c->cur[dst], c->cur[src], F(c), a));
+    touched(c);
+    return PTS_OK;
+}
+
+int pts_swap(pts_ctx* c, int a, int b) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = ensure_list(c, {a, b})) return rc;
+    int64_t x[3], y[3];
+    field_shape(c->nd, c->nx, c->ny, c->nz, a, x);
+    field_shape(c->nd, c->nx, c->ny, c->nz, b, y);
+    if (x[0] != y[0] || x[1] != y[1] || x[2] != y[2]) return fail(c, PTS_ERR_ARG, "swap: shape mismatch");
+    std::swap(c->cur[a], c->cur[b]);
+    std::swap(c->alt[a], c->alt[b]);
+    touched(c);
+    return PTS_OK;
+}
+
+int pts_sync(pts_ctx* c) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PTS_OK;
+}
+
+// ---- setup ------------------------------------------------------------------
+int pts_init_inclusion(pts_ctx* c, double lx, double ly, double lz, double mus0, double musi, double rad2, double ebg,
+                       double rin, double rout, int32_t igg) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    int st[16];
+    const int ns = state_fields(c, st);
+    for (int q = 0; q < ns; ++q)
+        if (int rc = ensure(c, st[q])) return rc;
+    if (int rc = ensure(c, PTS_MUS)) return rc;
+    if (c->g.body_force)
+        if (int rc = ensure(c, PTS_RHOG)) return rc;
+    if (c->g.ve) {
+        for (int f : {PTS_TXX_O, PTS_TYY_O, PTS_TXY_O})
+            if (int rc = ensure(c, f)) return rc;
+        if (c->nd == 3)
+            for (int f : {PTS_TZZ_O, PTS_TXZ_O, PTS_TYZ_O})
+                if (int rc = ensure(c, f)) return rc;
+    }
+    // zero all state and stress-history arrays (the scripts' @zeros)
+    for (int f = 0; f < PTS_NFIELDS; ++f)
+        if (c->cur[f]) HIPCHK(c, hipMemsetAsync(c->cur[f], 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    HIPCHK(c, launch_init_inclusion(c->stream, F(c), c->nd, lx, ly, lz, mus0, musi, rad2, ebg, rin, rout, igg));
+    touched(c);
+    return PTS_OK;
+}
+
+int pts_smooth(pts_ctx* c, int dst, int src, double fact) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure_list(c, {dst, src})) return rc;
+    int64_t s[3];
+    field_shape(c->nd, c->nx, c->ny, c->nz, src, s);
+    HIPCHK(c, launch_smooth(c->stream, F(c), c->nd, c->cur[dst], c->cur[src], s, fact));
+    touched(c);
+    return PTS_OK;
+}
+
+int pts_maxloc(pts_ctx* c, int dst, int src) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure_list(c, {dst, src})) return rc;
+    int64_t s[3];
+    field_shape(c->nd, c->nx, c->ny, c->nz, src, s);
+    HIPCHK(c, launch_maxloc(c->stream, F(c), c->nd, c->cur[dst], c->cur[src], s));
+    touched(c);
+    return PTS_OK;
+}
+
+static int bc_dim(pts_ctx* c, int f, int dim) {
+    if (!c) return PTS_ERR_ARG;
+    if (dim == 2 && c->nd != 3) return fail(c, PTS_ERR_STATE, "bc_z on a 2D grid");
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure(c, f)) return rc;
+    int64_t s[3];
+    field_shape(c->nd, c->nx, c->ny, c->nz, f, s);
+    HIPCHK(c, launch_bc(c->stream, F(c), c->cur[f], s, dim));
+    touched(c);
+    return PTS_OK;
+}
+int pts_bc_x(pts_ctx* c, int f) { return bc_dim(c, f, 0); }
+int pts_bc_y(pts_ctx* c, int f) { return bc_dim(c, f, 1); }
+int pts_bc_z(pts_ctx* c, int f) { return bc_dim(c, f, 2); }
+
+int pts_iter_params(pts_ctx* c, double Vp, double G, double dt, double Re, double r, double ml) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure_list(c, {PTS_MUSTAU, PTS_GDTAU, PTS_DTAU_RHO})) return rc;
+    HIPCHK(c, launch_iter_params(c->stream, F(c), c->nd, c->g.ve != 0, Vp, G, dt, Re, r, ml));
+    return PTS_OK;
+}
+
+int pts_assign_tau(pts_ctx* c) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    const int src2[] = {PTS_TXX, PTS_TYY, PTS_TXY}, dst2[] = {PTS_TXX_O, PTS_TYY_O, PTS_TXY_O};
+    const int src3[] = {PTS_TXX, PTS_TYY, PTS_TZZ, PTS_TXY, PTS_TXZ, PTS_TYZ};
+    const int dst3[] = {PTS_TXX_O, PTS_TYY_O, PTS_TZZ_O, PTS_TXY_O, PTS_TXZ_O, PTS_TYZ_O};
+    const int n = c->nd == 2 ? 3 : 6;
+    for (int q = 0; q < n; ++q) {
+        const int s = c->nd == 2 ? src2[q] : src3[q], d = c->nd == 2 ? dst2[q] : dst3[q];
+        if (int rc = ensure_list(c, {s, d})) return rc;
+        int64_t sh[3];
+        field_shape(c->nd, c->nx, c->ny, c->nz, s, sh);
+        HIPCHK(c, launch_copy(c->stream, c->cur[d], c->cur[s], F(c), sh));
+    }
+    return PTS_OK;
+}
+
+// ---- per-kernel reference API --------------------------------------------
+static int need_iter_fields(pts_ctx* c, bool with_o) {
+    int st[16];
+    const int ns = state_fields(c, st);
+    for (int q = 0; q < ns; ++q)
+        if (int rc = ensure(c, st[q])) return rc;
+    if (int rc = ensure_list(c, {PTS_MUS, PTS_GDTAU, PTS_DTAU_RHO})) return rc;
+    if (c->g.body_force)
+        if (int rc = ensure(c, PTS_RHOG)) return rc;
+    if (with_o && c->g.ve) {
+        for (int f : {PTS_TXX_O, PTS_TYY_O, PTS_TXY_O})
+            if (int rc = ensure(c, f)) return rc;
+        if (c->nd == 3)
+            for (int f : {PTS_TZZ_O, PTS_TXZ_O, PTS_TYZ_O})
+                if (int rc = ensure(c, f)) return rc;
+    }
+    return PTS_OK;
+}
+
+int pts_compute_P(pts_ctx* c, double r, double dx, double dy, double dz) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = need_iter_fields(c, false)) return rc;
+    if (int rc = ensure(c, PTS_DIVV)) return rc;
+    StepScalars s = scal(r, 1.0, 1.0, dx, dy, dz);
+    HIPCHK(c, c->nd == 2 ? launch2_compute_P(c->stream, F(c), s) : launch3_compute_P(c->stream, F(c), s));
+    return PTS_OK;
+}
+
+int pts_compute_tau(pts_ctx* c, double G, double dt, double dx, double dy, double dz) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = need_iter_fields(c, true)) return rc;
+    StepScalars s = scal(0.0, G, dt, dx, dy, dz);
+    const bool ve = c->g.ve != 0;
+    HIPCHK(c, c->nd == 2 ? launch2_compute_tau(c->stream, F(c), s, ve) : launch3_compute_tau(c->stream, F(c), s, ve));
+    return PTS_OK;
+}
+
+int pts_compute_dV(pts_ctx* c, double dx, double dy, double dz) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = need_iter_fields(c, false)) return rc;
+    if (int rc = ensure_list(c, {PTS_DVX, PTS_DVY})) return rc;
+    if (c->nd == 2) {
+        if (int rc = ensure_list(c, {PTS_RX, PTS_RY})) return rc;
+    } else {
+        if (int rc = ensure(c, PTS_DVZ)) return rc;
+    }
+    StepScalars s = scal(0.0, 1.0, 1.0, dx, dy, dz);
+    const bool bf = c->g.body_force != 0;
+    HIPCHK(c, c->nd == 2 ? launch2_compute_dV(c->stream, F(c), s, bf) : launch3_compute_dV(c->stream, F(c), s, bf));
+    return PTS_OK;
+}
+
+int pts_compute_V(pts_ctx* c) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = need_iter_fields(c, false)) return rc;
+    if (int rc = ensure_list(c, {PTS_DVX, PTS_DVY})) return rc;
+    if (c->nd == 3)
+        if (int rc = ensure(c, PTS_DVZ)) return rc;
+    HIPCHK(c, c->nd == 2 ? launch2_compute_V(c->stream, F(c)) : launch3_compute_V(c->stream, F(c)));
+    touched(c);
+    return PTS_OK;
+}
+
+static int res_common(pts_ctx* c, double dx, double dy, double dz, bool mul) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = need_iter_fields(c, false)) return rc;
+    if (int rc = ensure_list(c, {PTS_RX, PTS_RY, PTS_DIVV})) return rc;
+    if (c->nd == 3)
+        if (int rc = ensure(c, PTS_RZ)) return rc;
+    StepScalars s = mul ? scal(0.0, 1.0, 1.0, 1.0 / dx, 1.0 / dy, 1.0 / dz) : scal(0.0, 1.0, 1.0, dx, dy, dz);
+    if (mul) {  // callers pass _dx; keep the given reciprocal exactly
+        s._dx = dx; s._dy = dy; s._dz = dz;
+    }
+    const bool bf = c->g.body_force != 0;
+    HIPCHK(c, c->nd == 2 ? launch2_compute_Res(c->stream, F(c), s, bf, mul)
+                         : launch3_compute_Res(c->stream, F(c), s, bf, mul));
+    return PTS_OK;
+}
+int pts_compute_Res(pts_ctx* c, double dx, double dy, double dz) { return res_common(c, dx, dy, dz, false); }
+int pts_compute_Res_perf(pts_ctx* c, double _dx, double _dy, double _dz) { return res_common(c, _dx, _dy, _dz, true); }
+
+int pts_compute_Pt_tau(pts_ctx* c, double r, double G, double dt, double _dx, double _dy, double _dz) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = need_iter_fields(c, true)) return rc;
+    StepScalars s = scal(r, G, dt, 1.0, 1.0, 1.0);
+    s._dx = _dx; s._dy = _dy; s._dz = _dz;
+    const bool ve = c->g.ve != 0;
+    HIPCHK(c, c->nd == 2 ? launch2_compute_Pt_tau(c->stream, F(c), s, ve)
+                         : launch3_compute_Pt_tau(c->stream, F(c), s, ve, true));
+    return PTS_OK;
+}
+
+int pts_compute_V_perf(pts_ctx* c, double _dx, double _dy, double _dz) {
+    if (!c) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = need_iter_fields(c, false)) return rc;
+    StepScalars s = scal(0.0, 1.0, 1.0, 1.0, 1.0, 1.0);
+    s._dx = _dx; s._dy = _dy; s._dz = _dz;
+    const bool bf = c->g.body_force != 0;
+    HIPCHK(c, c->nd == 2 ? launch2_compute_V_perf(c->stream, F(c), s, bf)
+                         : launch3_compute_V_perf(c->stream, F(c), s, bf, true, false));
+    touched(c);
+    return PTS_OK;
+}
+
+// ---- norms -----------------------------------------------------------------
+static void unpack_norms(const pts_ctx* c, const double* v, pts_norms* o) {
+    o->Vmin = v[0]; o->Vmax = v[1]; o->Pmin = v[2]; o->Pmax = v[3];
+    for (int q = 0; q < 4; ++q) {
+        o->sumsq[q] = v[4 + q];
+        o->maxabs[q] = v[8 + q];
+    }
+    const int fs[4] = {PTS_RX, PTS_RY, PTS_RZ, PTS_DIVV};
+    for (int q = 0; q < 4; ++q) {
+        if (q == 2 && c->nd == 2) { o->len[q] = 0; continue; }
+        int64_t s[3];
+        field_shape(c->nd, c->nx, c->ny, c->nz, fs[q], s);
+        o->len[q] = s[0] * s[1] * s[2];
+    }
+}
+
+int pts_norms_stored(pts_ctx* c, pts_norms* out) {
+    if (!c || !out) return PTS_ERR_ARG;
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure_list(c, {PTS_VX, PTS_PT, PTS_RX, PTS_RY, PTS_DIVV})) return rc;
+    if (c->nd == 3)
+        if (int rc = ensure(c, PTS_RZ)) return rc;
+    HIPCHK(c, launch_norms_stored(c->stream, F(c), c->nd, c->red_partials, c->red_out));
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, sizeof(double) * kRedVals, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unpack_norms(c, c->h_red, out);
+    return PTS_OK;
+}
+
+double pts_err_from_norms(const pts_norms* n, int32_t ndim, double lx, int32_t kind, double comp[4]) {
+    double err = -INFINITY;
+    bool nan = false;
+    for (int q = 0; q < 4; ++q) {
+        double v = 0.0;
+        if (!(q == 2 && ndim == 2)) {
+            const double den = q == 3 ? (n->Vmax - n->Vmin) : (n->Pmax - n->Pmin);
+            if (kind == 1) v = n->maxabs[q] / den * lx;
+            else v = std::sqrt(n->sumsq[q]) / den * lx / std::sqrt((double)n->len[q]);
+            if (v != v) nan = true;
+            if (v > err) err = v;
+        }
+        if (comp) comp[q] = v;
+    }
+    return nan ? NAN : err;
+}
+
+// ---- device-resident PT iteration -------------------------------------------
+static bool single_pass(const pts_ctx* c) { return c->nd == 2; }
+
+static int prepare_iter(pts_ctx* c) {
+    if (int rc = need_iter_fields(c, true)) return rc;
+    if (single_pass(c)) {
+        int st[16];
+        const int ns = state_fields(c, st);
+        for (int q = 0; q < ns; ++q)
+            if (int rc = ensure_alt(c, st[q])) return rc;
+        if (!c->alt_synced) {
+            for (int f : kVel)
+                if (field_valid(c, f))
+                    HIPCHK(c, hipMemcpyAsync(c->alt[f], c->cur[f], (size_t)c->nalloc * sizeof(double),
+                                             hipMemcpyDeviceToDevice, c->stream));
+            c->alt_synced = true;
+        }
+    }
+    return PTS_OK;
+}
+
+static void swap_state(pts_ctx* c) {
+    int st[16];
+    const int ns = state_fields(c, st);
+    for (int q = 0; q < ns; ++q) std::swap(c->cur[st[q]], c->alt[st[q]]);
+    c->parity ^= 1;
+}
+
+// Enqueue n iterations; for the 3D divide form, the last one stores ∇V of
+// the pre-update velocities when store_last_dv is set.
+static int enqueue_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_last_dv) {
+    const bool ve = c->g.ve != 0, mul = c->g.fd_mode == PTS_FD_MUL, bf = c->g.body_force != 0;
+    for (int64_t q = 0; q < n; ++q) {
+        if (single_pass(c)) {
+            HIPCHK(c, launch2_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->rows_per_block));
+            swap_state(c);
+        } else {
+            if (store_last_dv && q == n - 1 && !mul) {
+                if (int rc = ensure(c, PTS_DIVV)) return rc;
+                HIPCHK(c, launch3_compute_P(c->stream, F(c), s));
+                HIPCHK(c, launch3_compute_tau(c->stream, F(c), s, ve));
+            } else {
+                HIPCHK(c, launch3_compute_Pt_tau(c->stream, F(c), s, ve, mul));
+            }
+            HIPCHK(c, launch3_compute_V_perf(c->stream, F(c), s, bf, mul, true));
+        }
+    }
+    return PTS_OK;
+}
+
+static int run_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_last_dv, bool use_graph) {
+    if (n <= 0) return PTS_OK;
+    if (!use_graph || !single_pass(c) || n < 2) return enqueue_iters(c, s, n, store_last_dv);
+    // Capture n single-pass iterations once per (n, parity) and replay.
+    auto key = std::make_pair(n, c->parity);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        hipGraph_t graph;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_iters(c, s, n, store_last_dv);
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        hipGraphExec_t ex;
+        e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        // capture already swapped the buffers n times: restore, then replay
+        for (int64_t q = 0; q < n; ++q) swap_state(c);
+        c->graphs[key] = ex;
+        it = c->graphs.find(key);
+    }
+    HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+    for (int64_t q = 0; q < n; ++q) swap_state(c);
+    return PTS_OK;
+}
+
+int pts_pt_iterate(pts_ctx* c, const pts_pt_params* p, int64_t niter) {
+    if (!c || !p || niter < 0) return fail(c, PTS_ERR_ARG, "bad argument");
+    if (int rc = setdev(c)) return rc;
+    if (int rc = prepare_iter(c)) return rc;
+    StepScalars s = scal(p->r, p->G, p->dt, p->dx, p->dy, c->nd == 3 ? p->dz : 1.0);
+    return run_iters(c, s, niter, false, false);
+}
+
+static int live_norms(pts_ctx* c, const StepScalars& s, pts_norms* out) {
+    const bool mul = c->g.fd_mode == PTS_FD_MUL, bf = c->g.body_force != 0;
+    if (c->nd == 3 && !mul) {
+        // R live, ∇V stored by the last iteration's compute_P
+        if (int rc = ensure_list(c, {PTS_RX, PTS_RY, PTS_RZ})) return rc;
+        HIPCHK(c, launch3_compute_Res(c->stream, F(c), s, bf, false));
+        HIPCHK(c, launch_norms_stored(c->stream, F(c), c->nd, c->red_partials, c->red_out));
+    } else {
+        // divide form (4-kernel scripts): ∇V of the velocities before the last
+        // update, which the ping-pong partner still holds
+        Fields fv = (mul || !single_pass(c)) ? F(c) : fields_of(c, c->alt);
+        HIPCHK(c, launch_norms_live(c->stream, F(c), fv, s, c->nd, bf, mul, c->red_partials, c->red_out));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, sizeof(double) * kRedVals, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unpack_norms(c, c->h_red, out);
+    return PTS_OK;
+}
+
+int pts_pt_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp, pts_solve_result* res) {
+    if (!c || !p || !sp || !res || sp->nout < 1) return fail(c, PTS_ERR_ARG, "bad argument");
+    if (int rc = setdev(c)) return rc;
+    if (int rc = prepare_iter(c)) return rc;
+    StepScalars s = scal(p->r, p->G, p->dt, p->dx, p->dy, c->nd == 3 ? p->dz : 1.0);
+    std::memset(res, 0, sizeof(*res));
+    int64_t iter = 0;
+    double err = 2.0 * sp->eps;
+    // while err > ε && iter <= iterMax (Stokes2D_ve.jl:153)
+    while (err > sp->eps && iter <= sp->iter_max) {
+        int64_t n = sp->nout - (iter % sp->nout);
+        n = std::min<int64_t>(n, sp->iter_max + 1 - iter);
+        const bool check = ((iter + n) % sp->nout) == 0;
+        if (int rc = run_iters(c, s, n, check, sp->use_graph != 0)) return rc;
+        iter += n;
+        if (check) {
+            pts_norms nr;
+            if (int rc = live_norms(c, s, &nr)) return rc;
+            err = pts_err_from_norms(&nr, c->nd, p->lx, sp->norm_kind, res->comp);
+            res->nchecks += 1;
+            if (err != err) {
+                res->iters = iter;
+                res->err = err;
+                return fail(c, PTS_ERR_NAN, "NaN in residual norm after " + std::to_string(iter) + " iterations");
+            }
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    res->iters = iter;
+    res->err = err;
+    res->converged = err <= sp->eps ? 1 : 0;
+    if (!res->converged) return fail(c, PTS_ERR_NOCONV, "iter_max reached with err > eps");
+    return PTS_OK;
+}
+
+}  // extern "C"

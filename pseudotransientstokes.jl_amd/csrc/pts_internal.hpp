@@ -1,0 +1,133 @@
+// pts_internal.hpp -- shared definitions of libpts (gfx950 HIP).
+//
+// Device layout: every field of a context lives in its own allocation with
+// the SAME strides: element (i,j,k) of any field is at i + sx*j + sxy*k,
+// sx = nx+1 rounded up to 32 elements (256 B rows), sxy = sx*(ny+1).  A field
+// only uses its reference extent inside that box (e.g. τxy uses
+// (nx-1)x(ny-1)); one index therefore addresses every array of a stencil, so
+// a kernel computes one offset per cell instead of one per array.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/pts.h"
+
+namespace pts {
+
+constexpr int kNF = PTS_NFIELDS;
+
+// All device pointers of a context plus the shared strides, passed by value.
+struct Fields {
+    double* p[kNF];
+    int sx;    // row stride (elements)
+    int sxy;   // plane stride (elements)
+    int nx, ny, nz;
+};
+
+// Reference shape of a field for the grid (shape[2] = 1 in 2D).
+void field_shape(int ndim, int nx, int ny, int nz, int field, int64_t out[3]);
+
+struct Ctx;
+
+// ---- shuffles across the 64-wide wavefront ---------------------------------
+// from_left(v)[lane] = v[lane-1]; from_right(v)[lane] = v[lane+1].  Lanes at
+// the wave ends receive 0.  DPP wave_shr/wave_shl (GFX9 family) move one
+// dword per instruction without touching LDS.
+__device__ __forceinline__ double from_left(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_right(double v) {
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Scaled difference in the two flavours of the reference: (a-b)/dx in the
+// 4-kernel scripts, _dx*(a-b) in the *_perf scripts.
+template <bool MUL>
+__device__ __forceinline__ double dfd(double diff, double h, double ih) {
+    if constexpr (MUL) return ih * diff;
+    else return diff / h;
+}
+
+// Normal-stress update τ ← (τ + τ_o·Gr + 2·Gdτ·ε̇)/(1 + Gdτ/μ + Gr), Gr = Gdτ/(G·dt)
+// (Stokes2D_ve.jl:51-52, Stokes3D_ve_perf.jl:71-73); viscous drops τ_o and Gr.
+// e is the already-scaled strain rate.
+template <bool VE>
+__device__ __forceinline__ double tau_upd(double t, double to, double g, double m, double Gdt, double e) {
+    if constexpr (VE) {
+        double Gr = g / Gdt;
+        return (t + to * Gr + 2.0 * g * e) / (1.0 + g / m + Gr);
+    } else {
+        return (t + 2.0 * g * e) / (1.0 + g / m);
+    }
+}
+
+// Stokes2D.jl:40-41 writes the viscous normal stress as
+// (τ + 2.0*Gdτ*d/dx)/(Gdτ/μ + 1.0), i.e. ((2·Gdτ)·d)/dx.
+__device__ __forceinline__ double tau_visc2d_div(double t, double g, double m, double d, double h) {
+    return (t + 2.0 * g * d / h) / (g / m + 1.0);
+}
+
+// Scalars of one PT iteration.
+struct StepScalars {
+    double r, Gdt;
+    double dx, dy, dz, _dx, _dy, _dz;
+};
+
+// ---- kernel launchers (defined in the .hip files) ---------------------------
+hipError_t launch_fill(hipStream_t s, double* p, double v, const Fields& f, const int64_t shape[3]);
+hipError_t launch_copy(hipStream_t s, double* dst, const double* src, const Fields& f, const int64_t shape[3]);
+hipError_t launch_init_inclusion(hipStream_t s, const Fields& f, int ndim, double lx, double ly, double lz,
+                                 double mus0, double musi, double rad2, double ebg, double rin, double rout,
+                                 int igg);
+hipError_t launch_smooth(hipStream_t s, const Fields& f, int ndim, double* dst, const double* src,
+                         const int64_t shape[3], double fact);
+hipError_t launch_maxloc(hipStream_t s, const Fields& f, int ndim, double* dst, const double* src,
+                         const int64_t shape[3]);
+hipError_t launch_bc(hipStream_t s, const Fields& f, double* a, const int64_t shape[3], int dim);
+hipError_t launch_iter_params(hipStream_t s, const Fields& f, int ndim, bool ve, double Vp, double G, double dt,
+                              double Re, double r, double ml);
+
+// 2D per-kernel API (in place, reference semantics)
+hipError_t launch2_compute_P(hipStream_t s, const Fields& f, const StepScalars& c);
+hipError_t launch2_compute_tau(hipStream_t s, const Fields& f, const StepScalars& c, bool ve);
+hipError_t launch2_compute_dV(hipStream_t s, const Fields& f, const StepScalars& c, bool bf);
+hipError_t launch2_compute_V(hipStream_t s, const Fields& f);
+hipError_t launch2_compute_Res(hipStream_t s, const Fields& f, const StepScalars& c, bool bf, bool mul);
+hipError_t launch2_compute_Pt_tau(hipStream_t s, const Fields& f, const StepScalars& c, bool ve);
+hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScalars& c, bool bf);
+// 2D single-pass iteration: reads `in`, writes `out` (ping-pong).
+hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
+                        bool mul, bool bf, int rows_per_block);
+
+// 3D
+hipError_t launch3_compute_P(hipStream_t s, const Fields& f, const StepScalars& c);
+hipError_t launch3_compute_tau(hipStream_t s, const Fields& f, const StepScalars& c, bool ve);
+hipError_t launch3_compute_dV(hipStream_t s, const Fields& f, const StepScalars& c, bool bf);
+hipError_t launch3_compute_V(hipStream_t s, const Fields& f);
+hipError_t launch3_compute_Res(hipStream_t s, const Fields& f, const StepScalars& c, bool bf, bool mul);
+hipError_t launch3_compute_Pt_tau(hipStream_t s, const Fields& f, const StepScalars& c, bool ve, bool mul);
+hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScalars& c, bool bf, bool mul,
+                                  bool fold_bc);
+// 3D single-pass iteration (ping-pong)
+hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
+                        bool mul, bool bf);
+
+// Reductions.  `red` holds kRedSlots partial slots per block.
+constexpr int kRedBlocks = 512;
+constexpr int kRedVals = 12;  // Vmin Vmax Pmin Pmax ss[4] ma[4]
+hipError_t launch_norms_stored(hipStream_t s, const Fields& f, int ndim, double* partials, double* out);
+// On-the-fly residual norms of the current state: R from Pt/τ, ∇V from the
+// V fields in `fv` (pass the pre-update V for the 4-kernel semantics).
+hipError_t launch_norms_live(hipStream_t s, const Fields& f, const Fields& fv, const StepScalars& c, int ndim,
+                             bool bf, bool mul, double* partials, double* out);
+
+}  // namespace pts

@@ -1,0 +1,403 @@
+// pts_step2d.hip -- 2D PT iteration kernels.
+//
+// Per-kernel reference API (in place, one launch per reference kernel) and
+// the single-pass iteration k2_iter used by pts_pt_iterate/pts_pt_solve.
+// Index convention: 0-based (i,j); every field at offset i + sx*j.
+#include "pts_internal.hpp"
+
+namespace pts {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+inline dim3 grid2(int n1, int n2) { return dim3((n1 + BX - 1) / BX, (n2 + BY - 1) / BY, 1); }
+
+#define IJ                                          \
+    const int i = blockIdx.x * BX + threadIdx.x;     \
+    const int j = blockIdx.y * BY + threadIdx.y;     \
+    const int nx = f.nx, ny = f.ny, sx = f.sx;       \
+    const int o = i + sx * j;                        \
+    (void)nx; (void)ny; (void)o;
+
+// compute_P! (Stokes2D.jl:33-37, Stokes2D_ve.jl:42-46)
+__global__ __launch_bounds__(BX* BY) void k2_P(Fields f, StepScalars c) {
+    IJ;
+    if (i < nx && j < ny) {
+        const double* Vx = f.p[PTS_VX];
+        const double* Vy = f.p[PTS_VY];
+        double d = (Vx[o + 1] - Vx[o]) / c.dx + (Vy[o + sx] - Vy[o]) / c.dy;
+        f.p[PTS_DIVV][o] = d;
+        f.p[PTS_PT][o] = f.p[PTS_PT][o] - c.r * f.p[PTS_GDTAU][o] * d;
+    }
+}
+
+// compute_τ! (Stokes2D.jl:39-44 viscous, Stokes2D_ve.jl:48-55 VE)
+template <bool VE>
+__global__ __launch_bounds__(BX* BY) void k2_tau(Fields f, StepScalars c) {
+    IJ;
+    const double* Vx = f.p[PTS_VX];
+    const double* Vy = f.p[PTS_VY];
+    const double* Gd = f.p[PTS_GDTAU];
+    const double* Mu = f.p[PTS_MUS];
+    if (i < nx && j < ny) {
+        double g = Gd[o], m = Mu[o];
+        double ex = Vx[o + 1] - Vx[o], ey = Vy[o + sx] - Vy[o];
+        double* txx = f.p[PTS_TXX];
+        double* tyy = f.p[PTS_TYY];
+        if constexpr (VE) {
+            txx[o] = tau_upd<true>(txx[o], f.p[PTS_TXX_O][o], g, m, c.Gdt, ex / c.dx);
+            tyy[o] = tau_upd<true>(tyy[o], f.p[PTS_TYY_O][o], g, m, c.Gdt, ey / c.dy);
+        } else {
+            txx[o] = tau_visc2d_div(txx[o], g, m, ex, c.dx);
+            tyy[o] = tau_visc2d_div(tyy[o], g, m, ey, c.dy);
+        }
+    }
+    if (i < nx - 1 && j < ny - 1) {
+        double ag = (Gd[o] + Gd[o + 1] + Gd[o + sx] + Gd[o + sx + 1]) * 0.25;
+        double am = (Mu[o] + Mu[o + 1] + Mu[o + sx] + Mu[o + sx + 1]) * 0.25;
+        double e = 0.5 * ((Vx[o + sx + 1] - Vx[o + 1]) / c.dy + (Vy[o + sx + 1] - Vy[o + sx]) / c.dx);
+        double* txy = f.p[PTS_TXY];
+        double to = VE ? f.p[PTS_TXY_O][o] : 0.0;
+        txy[o] = tau_upd<VE>(txy[o], to, ag, am, c.Gdt, e);
+    }
+}
+
+// Momentum residuals at (i,j) in either flavour (Stokes2D_ve.jl:70-71,
+// Stokes2D_ve_perf.jl:63-64), body force per Stokes2D_param.jl:59.
+template <bool MUL>
+__device__ __forceinline__ double res2_x(const Fields& f, const StepScalars& c, int o) {
+    const int sx = f.sx;
+    const double* txx = f.p[PTS_TXX];
+    const double* txy = f.p[PTS_TXY];
+    const double* Pt = f.p[PTS_PT];
+    return dfd<MUL>(txx[o + sx + 1] - txx[o + sx], c.dx, c._dx) + dfd<MUL>(txy[o + sx] - txy[o], c.dy, c._dy) -
+           dfd<MUL>(Pt[o + sx + 1] - Pt[o + sx], c.dx, c._dx);
+}
+template <bool MUL>
+__device__ __forceinline__ double res2_y(const Fields& f, const StepScalars& c, int o, bool bf) {
+    const int sx = f.sx;
+    const double* tyy = f.p[PTS_TYY];
+    const double* txy = f.p[PTS_TXY];
+    const double* Pt = f.p[PTS_PT];
+    double R = dfd<MUL>(tyy[o + sx + 1] - tyy[o + 1], c.dy, c._dy) + dfd<MUL>(txy[o + 1] - txy[o], c.dx, c._dx) -
+               dfd<MUL>(Pt[o + sx + 1] - Pt[o + 1], c.dy, c._dy);
+    if (bf) {
+        const double* rg = f.p[PTS_RHOG];
+        R = R - (rg[o + 1] + rg[o + sx + 1]) * 0.5;
+    }
+    return R;
+}
+
+// compute_dV! (Stokes2D.jl:46-52 stores Rx, Ry as well; Stokes2D_ve.jl:57-61)
+__global__ __launch_bounds__(BX* BY) void k2_dV(Fields f, StepScalars c, int bf) {
+    IJ;
+    const double* Rh = f.p[PTS_DTAU_RHO];
+    if (i < nx - 1 && j < ny - 2) {
+        double R = res2_x<false>(f, c, o);
+        f.p[PTS_RX][o] = R;
+        f.p[PTS_DVX][o] = R * ((Rh[o + sx] + Rh[o + sx + 1]) * 0.5);
+    }
+    if (i < nx - 2 && j < ny - 1) {
+        double R = res2_y<false>(f, c, o, bf != 0);
+        f.p[PTS_RY][o] = R;
+        f.p[PTS_DVY][o] = R * ((Rh[o + 1] + Rh[o + sx + 1]) * 0.5);
+    }
+}
+
+// compute_V! (Stokes2D.jl:54-58)
+__global__ __launch_bounds__(BX* BY) void k2_V(Fields f) {
+    IJ;
+    if (i < nx - 1 && j < ny - 2) f.p[PTS_VX][o + sx + 1] = f.p[PTS_VX][o + sx + 1] + f.p[PTS_DVX][o];
+    if (i < nx - 2 && j < ny - 1) f.p[PTS_VY][o + sx + 1] = f.p[PTS_VY][o + sx + 1] + f.p[PTS_DVY][o];
+}
+
+// compute_Res! (Stokes2D_ve.jl:69-73 divide form: Rx, Ry;
+// Stokes2D_ve_perf.jl:61-66 multiply form: ∇V, Rx, Ry)
+template <bool MUL>
+__global__ __launch_bounds__(BX* BY) void k2_Res(Fields f, StepScalars c, int bf) {
+    IJ;
+    if (MUL && i < nx && j < ny) {
+        const double* Vx = f.p[PTS_VX];
+        const double* Vy = f.p[PTS_VY];
+        f.p[PTS_DIVV][o] = c._dx * (Vx[o + 1] - Vx[o]) + c._dy * (Vy[o + sx] - Vy[o]);
+    }
+    if (i < nx - 1 && j < ny - 2) f.p[PTS_RX][o] = res2_x<MUL>(f, c, o);
+    if (i < nx - 2 && j < ny - 1) f.p[PTS_RY][o] = res2_y<MUL>(f, c, o, bf != 0);
+}
+
+// compute_Pt_τ! (Stokes2D_ve_perf.jl:47-53)
+template <bool VE>
+__global__ __launch_bounds__(BX* BY) void k2_Pt_tau(Fields f, StepScalars c) {
+    IJ;
+    const double* Vx = f.p[PTS_VX];
+    const double* Vy = f.p[PTS_VY];
+    const double* Gd = f.p[PTS_GDTAU];
+    const double* Mu = f.p[PTS_MUS];
+    if (i < nx && j < ny) {
+        double g = Gd[o], m = Mu[o];
+        double ex = c._dx * (Vx[o + 1] - Vx[o]), ey = c._dy * (Vy[o + sx] - Vy[o]);
+        f.p[PTS_PT][o] = f.p[PTS_PT][o] - c.r * g * (ex + ey);
+        double* txx = f.p[PTS_TXX];
+        double* tyy = f.p[PTS_TYY];
+        double txo = VE ? f.p[PTS_TXX_O][o] : 0.0, tyo = VE ? f.p[PTS_TYY_O][o] : 0.0;
+        txx[o] = tau_upd<VE>(txx[o], txo, g, m, c.Gdt, ex);
+        tyy[o] = tau_upd<VE>(tyy[o], tyo, g, m, c.Gdt, ey);
+    }
+    if (i < nx - 1 && j < ny - 1) {
+        double ag = 0.25 * (Gd[o] + Gd[o + 1] + Gd[o + sx] + Gd[o + sx + 1]);
+        double am = 0.25 * (Mu[o] + Mu[o + 1] + Mu[o + sx] + Mu[o + sx + 1]);
+        double e = 0.5 * (c._dy * (Vx[o + sx + 1] - Vx[o + 1]) + c._dx * (Vy[o + sx + 1] - Vy[o + sx]));
+        double* txy = f.p[PTS_TXY];
+        double to = VE ? f.p[PTS_TXY_O][o] : 0.0;
+        txy[o] = tau_upd<VE>(txy[o], to, ag, am, c.Gdt, e);
+    }
+}
+
+// compute_V! of the perf script (Stokes2D_ve_perf.jl:55-59)
+__global__ __launch_bounds__(BX* BY) void k2_V_perf(Fields f, StepScalars c, int bf) {
+    IJ;
+    const double* Rh = f.p[PTS_DTAU_RHO];
+    if (i < nx - 1 && j < ny - 2) {
+        double R = res2_x<true>(f, c, o);
+        f.p[PTS_VX][o + sx + 1] = f.p[PTS_VX][o + sx + 1] + R * (0.5 * (Rh[o + sx] + Rh[o + sx + 1]));
+    }
+    if (i < nx - 2 && j < ny - 1) {
+        double R = res2_y<true>(f, c, o, bf != 0);
+        f.p[PTS_VY][o + sx + 1] = f.p[PTS_VY][o + sx + 1] + R * (0.5 * (Rh[o + 1] + Rh[o + sx + 1]));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Single-pass iteration.  One 64-lane wave owns a strip of 62 output columns
+// (lane t handles column c = 62*strip - 1 + t; lanes 0 and 63 are halo) and a
+// chunk of rows, marching in y.  At row step J a lane computes
+//   * the new cell values Pt, τxx, τyy at (c, J)            (compute_Pt_τ!)
+//   * the new vertex τxy at (c-1, J-1)
+//   * the new Vx at face (c, J-1) and Vy at face (c, J)     (compute_V!)
+// with free-slip copies folded in (bc_y!(Vx), bc_x!(Vy)).  Neighbour values
+// across x come from DPP lane shifts; values from the previous row are kept
+// in registers.  The state is read from `in` and written to `out`, so every
+// array is streamed once per iteration.  Dirichlet (never-updated) entries of
+// Vx/Vy are not written: both ping-pong buffers hold them.
+// ---------------------------------------------------------------------------
+template <bool VE, bool MUL, bool BF>
+__global__ __launch_bounds__(64) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
+                                              int nstrips) {
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    // XCD-aware order: the hardware deals consecutive blocks to the 8 XCDs
+    // round-robin; remap so each XCD gets a contiguous run of tiles and the
+    // halo columns shared by neighbouring strips stay in one L2.
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    const int strip = b % nstrips, chunk = b / nstrips;
+    const int lane = threadIdx.x;
+    const int cx = strip * 62 - 1 + lane;
+    const int nx = in.nx, ny = in.ny, sx = in.sx;
+    const int jb = chunk * L;
+    const int je = min(jb + L, ny);
+    const bool own = lane >= 1 && lane <= 62;
+    const bool cin = cx >= 0 && cx < nx;       // cell column
+    const bool fin = cx >= 0 && cx <= nx;      // Vx face
+    const bool vin = cx >= 1 && cx <= nx - 1;  // vertex column cx-1 / interior Vx face
+    const bool vyin = cx >= 1 && cx <= nx - 2; // interior Vy column
+
+    const double* __restrict__ Pt0 = in.p[PTS_PT];
+    const double* __restrict__ txx0 = in.p[PTS_TXX];
+    const double* __restrict__ tyy0 = in.p[PTS_TYY];
+    const double* __restrict__ txy0 = in.p[PTS_TXY];
+    const double* __restrict__ Vx0 = in.p[PTS_VX];
+    const double* __restrict__ Vy0 = in.p[PTS_VY];
+    const double* __restrict__ txxo = in.p[PTS_TXX_O];
+    const double* __restrict__ tyyo = in.p[PTS_TYY_O];
+    const double* __restrict__ txyo = in.p[PTS_TXY_O];
+    const double* __restrict__ Gdp = in.p[PTS_GDTAU];
+    const double* __restrict__ Mup = in.p[PTS_MUS];
+    const double* __restrict__ Rhp = in.p[PTS_DTAU_RHO];
+    const double* __restrict__ rgp = in.p[PTS_RHOG];
+    double* __restrict__ Pt1 = out.p[PTS_PT];
+    double* __restrict__ txx1 = out.p[PTS_TXX];
+    double* __restrict__ tyy1 = out.p[PTS_TYY];
+    double* __restrict__ txy1 = out.p[PTS_TXY];
+    double* __restrict__ Vx1 = out.p[PTS_VX];
+    double* __restrict__ Vy1 = out.p[PTS_VY];
+
+    const double dx = c.dx, dy = c.dy, _dx = c._dx, _dy = c._dy, r = c.r, Gdt = c.Gdt;
+
+    // values carried from row J-1
+    double VxP = 0.0, GdP = 0.0, MuP = 0.0, GdPl = 0.0, MuPl = 0.0, RhP = 0.0, rgP = 0.0;
+    double Pt1P = 0.0, txx1P = 0.0, tyy1P = 0.0, txy1PP = 0.0;
+    const int j0 = max(jb - 1, 0), j1 = min(je, ny - 1);
+    double VyC = cin ? Vy0[cx + sx * j0] : 0.0;  // Vy0[c, J]
+
+    for (int J = j0; J <= j1; ++J) {
+        const int o = cx + sx * J;
+        double Pt0v = 0.0, tx0 = 0.0, ty0 = 0.0, txo = 0.0, tyo = 0.0, Gd = 0.0, Mu = 0.0, Rh = 0.0, VyN = 0.0,
+               rg = 0.0, Vx0v = 0.0, tz0 = 0.0, tzo = 0.0;
+        if (cin) {
+            Pt0v = Pt0[o];
+            tx0 = txx0[o];
+            ty0 = tyy0[o];
+            if constexpr (VE) {
+                txo = txxo[o];
+                tyo = tyyo[o];
+            }
+            Gd = Gdp[o];
+            Mu = Mup[o];
+            Rh = Rhp[o];
+            VyN = Vy0[o + sx];
+            if constexpr (BF) rg = rgp[o];
+        }
+        if (fin) Vx0v = Vx0[o];
+        const bool vtx = vin && J >= 1;
+        if (vtx) {
+            tz0 = txy0[o - 1 - sx];
+            if constexpr (VE) tzo = txyo[o - 1 - sx];
+        }
+        // lane exchanges (whole wave active)
+        const double VxR = from_right(Vx0v);
+        const double GdL = from_left(Gd);
+        const double MuL = from_left(Mu);
+        const double VyCL = from_left(VyC);
+        const double txx1PL = from_left(txx1P);
+        const double Pt1PL = from_left(Pt1P);
+        const double RhPL = from_left(RhP);
+
+        // cell (c, J)
+        const double dVx = VxR - Vx0v, dVy = VyN - VyC;
+        const double ex = dfd<MUL>(dVx, dx, _dx), ey = dfd<MUL>(dVy, dy, _dy);
+        const double Pt1n = Pt0v - r * Gd * (ex + ey);
+        double txx1n, tyy1n;
+        if constexpr (!VE && !MUL) {
+            txx1n = tau_visc2d_div(tx0, Gd, Mu, dVx, dx);
+            tyy1n = tau_visc2d_div(ty0, Gd, Mu, dVy, dy);
+        } else {
+            txx1n = tau_upd<VE>(tx0, txo, Gd, Mu, Gdt, ex);
+            tyy1n = tau_upd<VE>(ty0, tyo, Gd, Mu, Gdt, ey);
+        }
+        // vertex (c-1, J-1): 4-cell means over [c-1..c] x [J-1..J]
+        const double ag = (GdPl + GdP + GdL + Gd) * 0.25;
+        const double am = (MuPl + MuP + MuL + Mu) * 0.25;
+        const double e = 0.5 * (dfd<MUL>(Vx0v - VxP, dy, _dy) + dfd<MUL>(VyC - VyCL, dx, _dx));
+        const double txy1n = tau_upd<VE>(tz0, tzo, ag, am, Gdt, e);
+        const double txy1nR = from_right(txy1n);
+
+        // stores of the new cell/vertex values owned by this tile
+        if (own && cin && J >= jb && J < je) {
+            Pt1[o] = Pt1n;
+            txx1[o] = txx1n;
+            tyy1[o] = tyy1n;
+        }
+        const int Jm = J - 1;
+        if (own && vtx && Jm >= jb && Jm < je) txy1[o - 1 - sx] = txy1n;
+
+        // Vx at face (c, J-1)
+        if (own && Jm >= jb && Jm < je && Jm >= 1 && Jm <= ny - 2) {
+            if (vin) {
+                double R = dfd<MUL>(txx1P - txx1PL, dx, _dx) + dfd<MUL>(txy1n - txy1PP, dy, _dy) -
+                           dfd<MUL>(Pt1P - Pt1PL, dx, _dx);
+                double v = VxP + R * (0.5 * (RhPL + RhP));
+                Vx1[o - sx] = v;
+                if (Jm == 1) Vx1[o - 2 * sx] = v;           // bc_y!: Vx[:,1] = Vx[:,2]
+                if (Jm == ny - 2) Vx1[o] = v;               //        Vx[:,end] = Vx[:,end-1]
+            } else if (fin) {                               // Dirichlet faces c = 0, nx
+                if (Jm == 1) Vx1[o - 2 * sx] = VxP;
+                if (Jm == ny - 2) Vx1[o] = VxP;
+            }
+        }
+        // Vy at face (c, J)
+        if (own && J >= jb && J < je && J >= 1) {
+            if (vyin) {
+                double R = dfd<MUL>(tyy1n - tyy1P, dy, _dy) + dfd<MUL>(txy1nR - txy1n, dx, _dx) -
+                           dfd<MUL>(Pt1n - Pt1P, dy, _dy);
+                if constexpr (BF) R = R - (rgP + rg) * 0.5;
+                double v = VyC + R * (0.5 * (RhP + Rh));
+                Vy1[o] = v;
+                if (cx == 1) Vy1[o - 1] = v;                // bc_x!: Vy[1,:] = Vy[2,:]
+                if (cx == nx - 2) Vy1[o + 1] = v;           //        Vy[end,:] = Vy[end-1,:]
+            }
+        }
+        // Dirichlet rows of Vy (face rows 0 and ny): their x-copies
+        if (own && (cx == 1 || cx == nx - 2)) {
+            if (J == 0 && jb == 0) {
+                if (cx == 1) Vy1[o - 1] = VyC;
+                if (cx == nx - 2) Vy1[o + 1] = VyC;
+            }
+            if (J == ny - 1 && je == ny) {
+                if (cx == 1) Vy1[o + sx - 1] = VyN;
+                if (cx == nx - 2) Vy1[o + sx + 1] = VyN;
+            }
+        }
+        // carry
+        VxP = Vx0v;
+        VyC = VyN;
+        GdPl = GdL;
+        MuPl = MuL;
+        GdP = Gd;
+        MuP = Mu;
+        RhP = Rh;
+        rgP = rg;
+        Pt1P = Pt1n;
+        txx1P = txx1n;
+        tyy1P = tyy1n;
+        txy1PP = txy1n;
+    }
+}
+
+}  // namespace
+
+hipError_t launch2_compute_P(hipStream_t s, const Fields& f, const StepScalars& c) {
+    k2_P<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, c);
+    return hipGetLastError();
+}
+hipError_t launch2_compute_tau(hipStream_t s, const Fields& f, const StepScalars& c, bool ve) {
+    if (ve) k2_tau<true><<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, c);
+    else k2_tau<false><<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, c);
+    return hipGetLastError();
+}
+hipError_t launch2_compute_dV(hipStream_t s, const Fields& f, const StepScalars& c, bool bf) {
+    k2_dV<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, c, bf ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t launch2_compute_V(hipStream_t s, const Fields& f) {
+    k2_V<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f);
+    return hipGetLastError();
+}
+hipError_t launch2_compute_Res(hipStream_t s, const Fields& f, const StepScalars& c, bool bf, bool mul) {
+    if (mul) k2_Res<true><<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, c, bf ? 1 : 0);
+    else k2_Res<false><<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, c, bf ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t launch2_compute_Pt_tau(hipStream_t s, const Fields& f, const StepScalars& c, bool ve) {
+    if (ve) k2_Pt_tau<true><<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, c);
+    else k2_Pt_tau<false><<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, c);
+    return hipGetLastError();
+}
+hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScalars& c, bool bf) {
+    k2_V_perf<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, c, bf ? 1 : 0);
+    return hipGetLastError();
+}
+
+template <bool VE, bool MUL, bool BF>
+static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, int ns,
+                int nblocks) {
+    k2_iter<VE, MUL, BF><<<nblocks, 64, 0, s>>>(in, out, c, L, ns);
+}
+
+hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
+                        bool mul, bool bf, int L) {
+    if (L < 1) L = 1;
+    const int ns = (in.nx + 1 + 61) / 62;
+    const int nc = (in.ny + L - 1) / L;
+    const int nb = ns * nc;
+    if (ve) {
+        if (mul) { if (bf) go2<true, true, true>(s, in, out, c, L, ns, nb); else go2<true, true, false>(s, in, out, c, L, ns, nb); }
+        else { if (bf) go2<true, false, true>(s, in, out, c, L, ns, nb); else go2<true, false, false>(s, in, out, c, L, ns, nb); }
+    } else {
+        if (mul) { if (bf) go2<false, true, true>(s, in, out, c, L, ns, nb); else go2<false, true, false>(s, in, out, c, L, ns, nb); }
+        else { if (bf) go2<false, false, true>(s, in, out, c, L, ns, nb); else go2<false, false, false>(s, in, out, c, L, ns, nb); }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace pts

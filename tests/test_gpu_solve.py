@@ -1,0 +1,98 @@
+"""Whole reference scripts replayed on the GPU: the golden files, the
+oracle's fields and iteration counts, and the published counts."""
+import numpy as np
+import pytest
+
+from gpu_cases import STATE, compare, pair
+from _util import FIXTURES, PUBLISHED_ITERS, julia_isapprox, load_fixture, relmax, sample
+from ptstokes import StokesModel, PtsError
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("preset", list(FIXTURES))
+def test_gpu_reproduces_golden_file(preset):
+    stem, n = FIXTURES[preset]
+    P, X, idx = load_fixture(stem)
+    m, o = pair(preset, n=n)
+    out = m.run()
+    res = o.solve()
+    S = sample(out[-1], idx)
+    assert julia_isapprox(S, P)
+    assert np.array_equal(S, P)
+    assert m.ittot == res["ittot"]
+    assert m.iters_per_step == res["iters_per_step"]
+    # every field matches the oracle (V and τ are not in the golden files)
+    assert compare(m, o, STATE) == []
+    m.close()
+
+
+@pytest.mark.parametrize("preset,n,want", PUBLISHED_ITERS[:3])
+def test_gpu_matches_published_iteration_counts(preset, n, want):
+    m = StokesModel(preset, n=n).setup()
+    m.run()
+    assert m.ittot == want
+    m.close()
+
+
+def test_gpu_matches_published_iteration_count_3d():
+    preset, n, want = PUBLISHED_ITERS[3]
+    m = StokesModel(preset, n=n).setup()
+    m.run()
+    assert m.ittot == want
+    m.close()
+
+
+def test_graph_and_stream_solves_agree():
+    a = StokesModel("Stokes2D_ve", n=63, nt=2).setup()
+    b = StokesModel("Stokes2D_ve", n=63, nt=2).setup()
+    a.run(use_graph=True)
+    b.run(use_graph=False)
+    assert a.iters_per_step == b.iters_per_step
+    for f in ("Pt", "Vx", "Vy", "txx", "tyy", "txy"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_err_matches_oracle_at_each_check(kind):
+    m, o = pair("Stokes2D_ve_perf", n=47, nout=10, iter_max=59, norm_kind=kind)
+    res = m.step()
+    ref = o.solve()
+    assert res["iters"] == ref["ittot"] == 60
+    assert res["err"] == pytest.approx(ref["err"], rel=1e-12)
+    m.close()
+
+
+def test_nan_is_reported():
+    m = StokesModel("Stokes2D_ve_perf", n=31, nout=5, iter_max=50).setup()
+    Pt = m.get("Pt")
+    Pt[10, 10] = np.nan
+    m.ctx.upload("Pt", Pt)
+    with pytest.raises(PtsError) as e:
+        m.step()
+    assert e.value.code == -3
+    m.close()
+
+
+def test_iter_max_reached_is_flagged():
+    m = StokesModel("Stokes2D_ve", n=63, iter_max=100).setup()
+    res = m.step()
+    assert not res["converged"] and res["iters"] == 101
+    m.close()
+
+
+def test_bad_arguments_raise():
+    from ptstokes import Context
+    with pytest.raises(PtsError):
+        Context(2, 2, 10)
+    c = Context(2, 10, 10)
+    with pytest.raises(ValueError):
+        c.upload("Pt", np.zeros((10, 11)))  # wrong shape
+    with pytest.raises(PtsError):  # wrong count at the C boundary
+        buf = np.zeros(5)
+        c._chk(c.lib.pts_upload(c.h, 0, buf.ctypes.data_as(__import__("ptstokes")._lib._dp), 5))
+    with pytest.raises(PtsError):
+        c.bc_z("Pt")  # 3D kernel on a 2D grid
+    c.close()

@@ -1,0 +1,48 @@
+"""The C-ABI library loads and exports every entry point of include/pts.h."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import ptstokes
+from ptstokes import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pts.h")
+
+
+def header_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|double|const char\s*\*)\s*(pts_\w+)\s*\(", txt, re.M)))
+
+
+def test_header_declares_the_api():
+    syms = header_symbols()
+    for s in ("pts_create", "pts_destroy", "pts_compute_P", "pts_compute_tau", "pts_compute_dV",
+              "pts_compute_V", "pts_compute_Pt_tau", "pts_bc_x", "pts_bc_y", "pts_bc_z", "pts_pt_solve"):
+        assert s in syms
+
+
+def test_library_exports_every_header_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in header_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_header_symbol():
+    bound = {name for name, _, _ in _lib.SIGNATURES}
+    assert bound == set(header_symbols())
+
+
+def test_field_names_match_header_enum():
+    txt = open(HEADER).read()
+    body = txt[txt.index("typedef enum pts_field"):txt.index("} pts_field;")]
+    names = re.findall(r"PTS_([A-Z0-9_]+)", body)
+    names = [n for n in names if n != "NFIELDS"]
+    assert len(names) == len(_lib.FIELDS)
+
+
+def test_package_loads_native_library():
+    lib = ptstokes.load()
+    assert lib.pts_last_error(None) is not None
