@@ -37,7 +37,10 @@ struct pts_ctx {
     double* red_out = nullptr;
     double* h_red = nullptr;
     int64_t bytes = 0;
-    int rows_per_block = 128;
+    int rows_per_block = 128;   // 2D single-pass: rows per block
+    int planes_per_block = 64;  // 3D single-pass: z planes per block
+    int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
+    bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
     std::string err;
     // graphs of n single-pass iterations keyed by (n, parity of the start buffer)
     std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
@@ -203,6 +206,9 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     c->sxy = (int)sxy;
     c->nalloc = na;
     if (const char* e = std::getenv("PTS_ROWS_PER_BLOCK")) c->rows_per_block = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
     hipError_t e = hipSetDevice(g->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void**)&c->red_partials, sizeof(double) * kRedBlocks * kRedVals);
@@ -573,7 +579,7 @@ double pts_err_from_norms(const pts_norms* n, int32_t ndim, double lx, int32_t k
 }
 
 // ---- device-resident PT iteration -------------------------------------------
-static bool single_pass(const pts_ctx* c) { return c->nd == 2; }
+static bool single_pass(const pts_ctx* c) { return c->nd == 2 || !c->two_kernel_3d; }
 
 static int prepare_iter(pts_ctx* c) {
     if (int rc = need_iter_fields(c, true)) return rc;
@@ -606,7 +612,11 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store
     const bool ve = c->g.ve != 0, mul = c->g.fd_mode == PTS_FD_MUL, bf = c->g.body_force != 0;
     for (int64_t q = 0; q < n; ++q) {
         if (single_pass(c)) {
-            HIPCHK(c, launch2_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->rows_per_block));
+            if (c->nd == 2)
+                HIPCHK(c, launch2_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->rows_per_block));
+            else
+                HIPCHK(c, launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
+                                       c->tile_rows_3d));
             swap_state(c);
         } else {
             if (store_last_dv && q == n - 1 && !mul) {
@@ -659,7 +669,7 @@ int pts_pt_iterate(pts_ctx* c, const pts_pt_params* p, int64_t niter) {
 
 static int live_norms(pts_ctx* c, const StepScalars& s, pts_norms* out) {
     const bool mul = c->g.fd_mode == PTS_FD_MUL, bf = c->g.body_force != 0;
-    if (c->nd == 3 && !mul) {
+    if (c->nd == 3 && !mul && !single_pass(c)) {
         // R live, ∇V stored by the last iteration's compute_P
         if (int rc = ensure_list(c, {PTS_RX, PTS_RY, PTS_RZ})) return rc;
         HIPCHK(c, launch3_compute_Res(c->stream, F(c), s, bf, false));

@@ -76,6 +76,20 @@ __device__ __forceinline__ double tau_visc2d_div(double t, double g, double m, d
     return (t + 2.0 * g * d / h) / (g / m + 1.0);
 }
 
+// Write v at (a,b) on the tangential plane of a velocity component and at the
+// free-slip copies it feeds: index a (stride sa, extent na) and b (stride sb,
+// extent nb); a==1 also writes a=0, a==na-2 also writes a=na-1, same for b
+// (bc_y!/bc_z! etc., Stokes3D_ve_perf.jl:202-207; edge values end up equal to
+// the diagonal interior neighbour whatever the order of the copies).
+__device__ __forceinline__ void put_bc(double* V, int q, double v, int a, int na, int sa, int b, int nb, int sb) {
+    V[q] = v;
+    const int da = (a == 1) ? -1 : ((a == na - 2) ? 1 : 0);
+    const int db = (b == 1) ? -1 : ((b == nb - 2) ? 1 : 0);
+    if (da) V[q + da * sa] = v;
+    if (db) V[q + db * sb] = v;
+    if (da && db) V[q + da * sa + db * sb] = v;
+}
+
 // Scalars of one PT iteration.
 struct StepScalars {
     double r, Gdt;
@@ -119,7 +133,7 @@ hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
                                   bool fold_bc);
 // 3D single-pass iteration (ping-pong)
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
-                        bool mul, bool bf);
+                        bool mul, bool bf, int planes_per_block, int tile_rows);
 
 // Reductions.  `red` holds kRedSlots partial slots per block.
 constexpr int kRedBlocks = 512;

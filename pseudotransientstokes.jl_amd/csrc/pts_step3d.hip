@@ -187,20 +187,6 @@ __global__ __launch_bounds__(BX* BY) void k3_Pt_tau(Fields f, StepScalars c) {
     tau3_all<VE, MUL>(f, c, i, j, k, o);
 }
 
-// Write v at (a,b) on the tangential plane of a velocity component and at the
-// free-slip copies it feeds: index a (stride sa, extent na) and b (stride sb,
-// extent nb); a==1 also writes a=0, a==na-2 also writes a=na-1, same for b
-// (bc_y!/bc_z! etc., Stokes3D_ve_perf.jl:202-207; edge values end up equal to
-// the diagonal interior neighbour whatever the order of the copies).
-__device__ __forceinline__ void put_bc(double* V, int q, double v, int a, int na, int sa, int b, int nb, int sb) {
-    V[q] = v;
-    const int da = (a == 1) ? -1 : ((a == na - 2) ? 1 : 0);
-    const int db = (b == 1) ? -1 : ((b == nb - 2) ? 1 : 0);
-    if (da) V[q + da * sa] = v;
-    if (db) V[q + db * sb] = v;
-    if (da && db) V[q + da * sa + db * sb] = v;
-}
-
 // compute_V! of the perf script (perf:83-88).  Thread (I,J,K) indexes the
 // velocity arrays directly.  With FOLD the free-slip copies of the six BC
 // kernels are written by the thread that computes their source value;
@@ -288,10 +274,6 @@ hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
     if (mul) { if (fold_bc) k3_V_perf<true, true><<<g, b, 0, s>>>(f, c, ibf); else k3_V_perf<true, false><<<g, b, 0, s>>>(f, c, ibf); }
     else { if (fold_bc) k3_V_perf<false, true><<<g, b, 0, s>>>(f, c, ibf); else k3_V_perf<false, false><<<g, b, 0, s>>>(f, c, ibf); }
     return hipGetLastError();
-}
-
-hipError_t launch3_iter(hipStream_t, const Fields&, const Fields&, const StepScalars&, bool, bool, bool) {
-    return hipErrorNotSupported;
 }
 
 }  // namespace pts

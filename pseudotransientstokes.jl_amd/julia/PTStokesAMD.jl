@@ -1,0 +1,121 @@
+# PTStokesAMD.jl -- Julia binding of libpts (include/pts.h) for the reference
+# scripts of PseudoTransientStokes.jl.
+#
+# Each function replaces one `@parallel kernel!(arrays..., scalars...)` call of
+# the reference with a `ccall` into libpts; the arrays live in the context.
+# Julia is not available where this repository is built and tested, so this
+# file is exercised only through the same C ABI from Python (ptstokes).
+module PTStokesAMD
+
+const libpts = get(ENV, "PTS_LIB", joinpath(@__DIR__, "..", "libpts.so"))
+
+const FD_DIV = Int32(0)   # 4-kernel scripts: (a-b)/dx
+const FD_MUL = Int32(1)   # *_perf scripts:   _dx*(a-b)
+
+const FIELDS = (:Pt, :Vx, :Vy, :Vz, :τxx, :τyy, :τzz, :τxy, :τxz, :τyz,
+                :τxx_o, :τyy_o, :τzz_o, :τxy_o, :τxz_o, :τyz_o,
+                :Mus, :Mus2, :Musτ, :Gdτ, :dτ_Rho, :∇V, :Rx, :Ry, :Rz, :dVx, :dVy, :dVz, :ρg)
+fid(s::Symbol) = Int32(findfirst(==(s), FIELDS) - 1)
+
+struct Grid
+    ndim::Int32; nx::Int32; ny::Int32; nz::Int32
+    ve::Int32; fd_mode::Int32; body_force::Int32; device::Int32
+end
+
+struct PtParams
+    r::Float64; G::Float64; dt::Float64; dx::Float64; dy::Float64; dz::Float64; lx::Float64
+end
+
+struct SolveParams
+    eps::Float64; nout::Int64; iter_max::Int64; norm_kind::Int32; use_graph::Int32
+end
+
+mutable struct SolveResult
+    iters::Int64; err::Float64; comp::NTuple{4,Float64}; converged::Int32; nchecks::Int32
+end
+SolveResult() = SolveResult(0, 0.0, (0.0, 0.0, 0.0, 0.0), 0, 0)
+
+mutable struct Ctx
+    h::Ptr{Cvoid}
+    ndim::Int
+end
+
+lasterr(c) = unsafe_string(ccall((:pts_last_error, libpts), Cstring, (Ptr{Cvoid},), c === nothing ? C_NULL : c.h))
+chk(c, rc) = rc == 0 ? nothing : error("libpts error $rc: $(lasterr(c))")
+
+function Ctx(ndim, nx, ny, nz=1; ve=true, fd_mode=FD_MUL, body_force=false, device=0)
+    g = Ref(Grid(ndim, nx, ny, nz, ve, fd_mode, body_force, device))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    chk(nothing, ccall((:pts_create, libpts), Cint, (Ref{Grid}, Ref{Ptr{Cvoid}}), g, h))
+    c = Ctx(h[], ndim)
+    finalizer(x -> ccall((:pts_destroy, libpts), Cint, (Ptr{Cvoid},), x.h), c)
+    return c
+end
+
+function shape(c::Ctx, f::Symbol)
+    s = zeros(Int64, 3)
+    chk(c, ccall((:pts_field_shape, libpts), Cint, (Ptr{Cvoid}, Cint, Ptr{Int64}), c.h, fid(f), s))
+    return Tuple(s[1:c.ndim])
+end
+# Data.Array(A) / Array(A): host <-> device, column-major as in the scripts
+upload!(c::Ctx, f::Symbol, A::Array{Float64}) =
+    chk(c, ccall((:pts_upload, libpts), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Int64), c.h, fid(f), A, length(A)))
+function download(c::Ctx, f::Symbol)
+    A = zeros(Float64, shape(c, f))
+    chk(c, ccall((:pts_download, libpts), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Int64), c.h, fid(f), A, length(A)))
+    return A
+end
+copy!(c::Ctx, dst::Symbol, src::Symbol) = chk(c, ccall((:pts_copy, libpts), Cint, (Ptr{Cvoid}, Cint, Cint), c.h, fid(dst), fid(src)))
+swap!(c::Ctx, a::Symbol, b::Symbol) = chk(c, ccall((:pts_swap, libpts), Cint, (Ptr{Cvoid}, Cint, Cint), c.h, fid(a), fid(b)))
+
+# setup kernels
+init_inclusion!(c::Ctx, lx, ly, lz, μs0, μsi, rad2, εbg, ρg_in=0.0, ρg_out=0.0; igg=false) =
+    chk(c, ccall((:pts_init_inclusion, libpts), Cint,
+                 (Ptr{Cvoid}, Float64, Float64, Float64, Float64, Float64, Float64, Float64, Float64, Float64, Int32),
+                 c.h, lx, ly, lz, μs0, μsi, rad2, εbg, ρg_in, ρg_out, igg))
+smooth!(c::Ctx, A2::Symbol, A::Symbol, fact) =
+    chk(c, ccall((:pts_smooth, libpts), Cint, (Ptr{Cvoid}, Cint, Cint, Float64), c.h, fid(A2), fid(A), fact))
+compute_maxloc!(c::Ctx, A2::Symbol, A::Symbol) =
+    chk(c, ccall((:pts_maxloc, libpts), Cint, (Ptr{Cvoid}, Cint, Cint), c.h, fid(A2), fid(A)))
+bc_x!(c::Ctx, A::Symbol) = chk(c, ccall((:pts_bc_x, libpts), Cint, (Ptr{Cvoid}, Cint), c.h, fid(A)))
+bc_y!(c::Ctx, A::Symbol) = chk(c, ccall((:pts_bc_y, libpts), Cint, (Ptr{Cvoid}, Cint), c.h, fid(A)))
+bc_z!(c::Ctx, A::Symbol) = chk(c, ccall((:pts_bc_z, libpts), Cint, (Ptr{Cvoid}, Cint), c.h, fid(A)))
+compute_iter_params!(c::Ctx, Vpdτ, G, dt, Re, r, max_l) =
+    chk(c, ccall((:pts_iter_params, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64, Float64, Float64, Float64),
+                 c.h, Vpdτ, G, dt, Re, r, max_l))
+assign_τ!(c::Ctx) = chk(c, ccall((:pts_assign_tau, libpts), Cint, (Ptr{Cvoid},), c.h))
+
+# 4-kernel form (Stokes2D.jl, Stokes2D_ve.jl, Stokes3D_ve_multixpu.jl)
+compute_P!(c::Ctx, r, dx, dy, dz=1.0) =
+    chk(c, ccall((:pts_compute_P, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64, Float64), c.h, r, dx, dy, dz))
+compute_τ!(c::Ctx, G, dt, dx, dy, dz=1.0) =
+    chk(c, ccall((:pts_compute_tau, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64, Float64, Float64), c.h, G, dt, dx, dy, dz))
+compute_dV!(c::Ctx, dx, dy, dz=1.0) =
+    chk(c, ccall((:pts_compute_dV, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64), c.h, dx, dy, dz))
+compute_V!(c::Ctx) = chk(c, ccall((:pts_compute_V, libpts), Cint, (Ptr{Cvoid},), c.h))
+compute_Res!(c::Ctx, dx, dy, dz=1.0) =
+    chk(c, ccall((:pts_compute_Res, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64), c.h, dx, dy, dz))
+
+# fused perf form (Stokes2D_ve_perf.jl, Stokes3D_ve_perf.jl)
+compute_Pt_τ!(c::Ctx, r, G, dt, _dx, _dy, _dz=1.0) =
+    chk(c, ccall((:pts_compute_Pt_tau, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64, Float64, Float64, Float64),
+                 c.h, r, G, dt, _dx, _dy, _dz))
+compute_V_perf!(c::Ctx, _dx, _dy, _dz=1.0) =
+    chk(c, ccall((:pts_compute_V_perf, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64), c.h, _dx, _dy, _dz))
+compute_Res_perf!(c::Ctx, _dx, _dy, _dz=1.0) =
+    chk(c, ccall((:pts_compute_Res_perf, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64), c.h, _dx, _dy, _dz))
+
+# device-resident PT loop of one physical step
+function pt_solve!(c::Ctx, p::PtParams, ε, nout, iterMax; norm_kind=0, use_graph=true)
+    sp = Ref(SolveParams(ε, nout, iterMax, norm_kind, use_graph))
+    res = Ref(SolveResult())
+    rc = ccall((:pts_pt_solve, libpts), Cint, (Ptr{Cvoid}, Ref{PtParams}, Ref{SolveParams}, Ref{SolveResult}),
+               c.h, Ref(p), sp, res)
+    rc == -3 && error("NaN")   # the scripts' `if isnan(err) error("NaN") end`
+    rc in (0, -4) || chk(c, rc)
+    return res[]
+end
+pt_iterate!(c::Ctx, p::PtParams, n) =
+    chk(c, ccall((:pts_pt_iterate, libpts), Cint, (Ptr{Cvoid}, Ref{PtParams}, Int64), c.h, Ref(p), n))
+
+end # module

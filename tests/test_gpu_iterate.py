@@ -74,3 +74,28 @@ def test_upload_after_iterate_resyncs_dirichlet_values():
     o.iterate(3)
     assert compare(m, o, STATE) == []
     m.close()
+
+
+VARIANTS = [
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_3D_TWO_KERNEL": "1"}),
+    ("Stokes3D_ve_multixpu", dict(n=21), {"PTS_3D_TWO_KERNEL": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_3D_TILE_ROWS": "8"}),
+    ("Stokes3D_ve_perf", dict(nx=33, ny=40, nz=12), {"PTS_3D_TILE_ROWS": "16"}),
+    ("Stokes3D_ve_perf", dict(nx=20, ny=17, nz=19), {"PTS_PLANES_PER_BLOCK": "3"}),
+    ("Stokes3D_ve_multixpu", dict(n=16, rhog_in=-1.0), {"PTS_PLANES_PER_BLOCK": "1"}),
+    ("Stokes2D_ve_perf", dict(nx=90, ny=41), {"PTS_ROWS_PER_BLOCK": "5"}),
+    ("Stokes2D", dict(nx=64, ny=33), {"PTS_ROWS_PER_BLOCK": "1"}),
+]
+
+
+@pytest.mark.parametrize("preset,kw,env", VARIANTS, ids=[f"{p}-{list(e)[0]}={list(e.values())[0]}" for p, _, e in VARIANTS])
+def test_launch_variants_bit_exact(preset, kw, env, monkeypatch):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    m, o = pair(preset, **kw)
+    randomise(m, o, seed=17)
+    for n in (1, 4):
+        m.iterate(n)
+        o.iterate(n)
+        assert compare(m, o, STATE) == [], n
+    m.close()
