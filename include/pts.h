@@ -99,6 +99,12 @@ int pts_device_bytes(const pts_ctx* ctx, int64_t* bytes);
  * Vx, Vz (3D x-z shear), all other state fields zero. */
 int pts_init_inclusion(pts_ctx* ctx, double lx, double ly, double lz, double mus0, double musi,
                        double rad2, double ebg, double rhog_in, double rhog_out, int32_t ic_igg);
+/* Viscosity from a Gaussian random field (no counterpart in the reference):
+ * log10 Mus = clip(sigma*sqrt(2/n)*sum_m cos(kx*x + ky*y + kz*z + phi), lo, hi)
+ * at the cell centres, from n modes given as (kx, ky, kz, phi) quadruples;
+ * Mus = musi where Rad2 < rad2 (the centred inclusion). */
+int pts_init_grf_viscosity(pts_ctx* ctx, int32_t nmodes, const double* modes, double sigma, double lo, double hi,
+                           double lx, double ly, double lz, double musi, double rad2);
 /* smooth! (Stokes2D_ve.jl:18-21, Stokes3D_ve_perf.jl:24-27): dst[inn] = src[inn] + 1/4.1/fact*(d2x+d2y)
  * (1/6.1 in 3D). */
 int pts_smooth(pts_ctx* ctx, int dst, int src, double fact);

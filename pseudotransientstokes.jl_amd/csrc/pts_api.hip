@@ -37,7 +37,9 @@ struct pts_ctx {
     double* red_out = nullptr;
     double* h_red = nullptr;
     int64_t bytes = 0;
-    int rows_per_block = 128;   // 2D single-pass: rows per block
+    int rows_per_block = 128;   // 2D single-pass: rows per block (0: one round of resident waves)
+    int variant_2d = 1;         // 2D single-pass code variant (prefetch)
+    bool persist_2d = false;    // 2D single-pass: persistent waves looping over tiles
     int planes_per_block = 64;  // 3D single-pass: z planes per block
     int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
@@ -205,7 +207,9 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     }
     c->sxy = (int)sxy;
     c->nalloc = na;
-    if (const char* e = std::getenv("PTS_ROWS_PER_BLOCK")) c->rows_per_block = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("PTS_ROWS_PER_BLOCK")) c->rows_per_block = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("PTS_2D_VARIANT")) c->variant_2d = std::atoi(e);
+    if (const char* e = std::getenv("PTS_2D_PERSIST")) c->persist_2d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
@@ -356,6 +360,22 @@ int pts_init_inclusion(pts_ctx* c, double lx, double ly, double lz, double mus0,
         if (c->cur[f]) HIPCHK(c, hipMemsetAsync(c->cur[f], 0, (size_t)c->nalloc * sizeof(double), c->stream));
     HIPCHK(c, launch_init_inclusion(c->stream, F(c), c->nd, lx, ly, lz, mus0, musi, rad2, ebg, rin, rout, igg));
     touched(c);
+    return PTS_OK;
+}
+
+int pts_init_grf_viscosity(pts_ctx* c, int32_t nmodes, const double* modes, double sigma, double lo, double hi,
+                           double lx, double ly, double lz, double musi, double rad2) {
+    if (!c || !modes || nmodes < 1) return fail(c, PTS_ERR_ARG, "bad argument");
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure(c, PTS_MUS)) return rc;
+    double* dm = nullptr;
+    const size_t bytes = sizeof(double) * 4 * (size_t)nmodes;
+    HIPCHK(c, hipMalloc((void**)&dm, bytes));
+    hipError_t e = hipMemcpyAsync(dm, modes, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_grf(c->stream, F(c), c->nd, dm, nmodes, sigma, lo, hi, lx, ly, lz, musi, rad2);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dm);
+    if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("grf: ") + hipGetErrorString(e));
     return PTS_OK;
 }
 
@@ -613,7 +633,8 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store
     for (int64_t q = 0; q < n; ++q) {
         if (single_pass(c)) {
             if (c->nd == 2)
-                HIPCHK(c, launch2_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->rows_per_block));
+                HIPCHK(c, launch2_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->rows_per_block,
+                                       c->variant_2d, c->persist_2d));
             else
                 HIPCHK(c, launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
                                        c->tile_rows_3d));

@@ -107,6 +107,8 @@ hipError_t launch_smooth(hipStream_t s, const Fields& f, int ndim, double* dst, 
 hipError_t launch_maxloc(hipStream_t s, const Fields& f, int ndim, double* dst, const double* src,
                          const int64_t shape[3]);
 hipError_t launch_bc(hipStream_t s, const Fields& f, double* a, const int64_t shape[3], int dim);
+hipError_t launch_grf(hipStream_t s, const Fields& f, int ndim, const double* modes, int nmodes, double sigma,
+                      double lo, double hi, double lx, double ly, double lz, double musi, double rad2);
 hipError_t launch_iter_params(hipStream_t s, const Fields& f, int ndim, bool ve, double Vp, double G, double dt,
                               double Re, double r, double ml);
 
@@ -119,8 +121,12 @@ hipError_t launch2_compute_Res(hipStream_t s, const Fields& f, const StepScalars
 hipError_t launch2_compute_Pt_tau(hipStream_t s, const Fields& f, const StepScalars& c, bool ve);
 hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScalars& c, bool bf);
 // 2D single-pass iteration: reads `in`, writes `out` (ping-pong).
+// rows_per_block < 1 sizes the row chunks so that the tiles fill about one
+// round of resident waves; persist caps the launch at that many waves, each
+// looping over tiles.  variant: 0 plain, 1 prefetch, 2 prefetch + non-temporal
+// stores, 3 non-temporal stores.
 hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
-                        bool mul, bool bf, int rows_per_block);
+                        bool mul, bool bf, int rows_per_block, int variant, bool persist);
 
 // 3D
 hipError_t launch3_compute_P(hipStream_t s, const Fields& f, const StepScalars& c);

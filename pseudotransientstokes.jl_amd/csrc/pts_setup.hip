@@ -175,6 +175,40 @@ __global__ void k_iter_params(Fields f, int ndim, int ve, double Vp, double G, d
     }
 }
 
+// Gaussian random field by random Fourier modes: f(x) = sigma*sqrt(2/N) *
+// sum_m cos(k_m . x + phi_m) at the cell centres; Mus = 10^clip(f, lo, hi)
+// outside the inclusion (Rad2 < rad2), musi inside.  The modes (kx, ky, kz,
+// phi per mode) are staged in LDS by each block.
+constexpr int kGrfChunk = 512;
+__global__ __launch_bounds__(256) void k_grf(Fields f, int ndim, const double* __restrict__ modes, int nmodes,
+                                             double amp, double lo, double hi, double dx, double dy, double dz,
+                                             double lx, double ly, double lz, double musi, double rad2) {
+    __shared__ double sm[kGrfChunk * 4];
+    const int nx = f.nx, ny = f.ny, nz = ndim == 3 ? f.nz : 1;
+    const int64_t tot = (int64_t)nx * ny * nz;
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int i = 0, j = 0, k = 0;
+    if (q < tot) unflat(q, nx, ny, i, j, k);
+    const double x = (double)i * dx + 0.5 * dx, y = (double)j * dy + 0.5 * dy, z = (double)k * dz + 0.5 * dz;
+    double acc = 0.0;
+    for (int m0 = 0; m0 < nmodes; m0 += kGrfChunk) {
+        const int nm = min(kGrfChunk, nmodes - m0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < nm * 4; t += blockDim.x) sm[t] = modes[(int64_t)m0 * 4 + t];
+        __syncthreads();
+        for (int m = 0; m < nm; ++m) {
+            double ph = sm[4 * m] * x + sm[4 * m + 1] * y + sm[4 * m + 3];
+            if (ndim == 3) ph += sm[4 * m + 2] * z;
+            acc += cos(ph);
+        }
+    }
+    if (q >= tot) return;
+    const double xr = x - 0.5 * lx, yr = y - 0.5 * ly, zr = ndim == 3 ? z - 0.5 * lz : 0.0;
+    const double r2 = xr * xr + yr * yr + zr * zr;
+    const double lg = fmin(fmax(amp * acc, lo), hi);
+    f.p[PTS_MUS][i + f.sx * (int64_t)j + (int64_t)f.sxy * k] = r2 < rad2 ? musi : pow(10.0, lg);
+}
+
 int grid_for(int64_t tot) {
     int64_t g = (tot + kBlock - 1) / kBlock;
     if (g > 65536) g = 65536;
@@ -229,6 +263,16 @@ hipError_t launch_bc(hipStream_t s, const Fields& f, double* a, const int64_t sh
     if (n < 2) return hipSuccess;
     int64_t tot = sh[0] * sh[1] * sh[2] / n;
     k_bc<<<grid_for(tot), kBlock, 0, s>>>(a, sh[0], sh[1], sh[2], f.sx, f.sxy, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_grf(hipStream_t s, const Fields& f, int ndim, const double* modes, int nmodes, double sigma,
+                      double lo, double hi, double lx, double ly, double lz, double musi, double rad2) {
+    const int64_t tot = (int64_t)f.nx * f.ny * (ndim == 3 ? f.nz : 1);
+    const double amp = sigma * sqrt(2.0 / nmodes);
+    const double dx = lx / f.nx, dy = ly / f.ny, dz = ndim == 3 ? lz / f.nz : 1.0;
+    k_grf<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(f, ndim, modes, nmodes, amp, lo, hi, dx, dy, dz, lx, ly, lz,
+                                                        musi, rad2);
     return hipGetLastError();
 }
 

@@ -3,6 +3,8 @@
 // Per-kernel reference API (in place, one launch per reference kernel) and
 // the single-pass iteration k2_iter used by pts_pt_iterate/pts_pt_solve.
 // Index convention: 0-based (i,j); every field at offset i + sx*j.
+#include <algorithm>
+
 #include "pts_internal.hpp"
 
 namespace pts {
@@ -181,166 +183,227 @@ __global__ __launch_bounds__(BX* BY) void k2_V_perf(Fields f, StepScalars c, int
 // array is streamed once per iteration.  Dirichlet (never-updated) entries of
 // Vx/Vy are not written: both ping-pong buffers hold them.
 // ---------------------------------------------------------------------------
-template <bool VE, bool MUL, bool BF>
-__global__ __launch_bounds__(64) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
-                                              int nstrips) {
-    int b = blockIdx.x;
-    const int nb = gridDim.x;
-    // XCD-aware order: the hardware deals consecutive blocks to the 8 XCDs
-    // round-robin; remap so each XCD gets a contiguous run of tiles and the
-    // halo columns shared by neighbouring strips stay in one L2.
-    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-    const int strip = b % nstrips, chunk = b / nstrips;
-    const int lane = threadIdx.x;
-    const int cx = strip * 62 - 1 + lane;
-    const int nx = in.nx, ny = in.ny, sx = in.sx;
-    const int jb = chunk * L;
-    const int je = min(jb + L, ny);
-    const bool own = lane >= 1 && lane <= 62;
-    const bool cin = cx >= 0 && cx < nx;       // cell column
-    const bool fin = cx >= 0 && cx <= nx;      // Vx face
-    const bool vin = cx >= 1 && cx <= nx - 1;  // vertex column cx-1 / interior Vx face
-    const bool vyin = cx >= 1 && cx <= nx - 2; // interior Vy column
+// Inputs of one row step (cell row J, vertex row J-1).
+struct Row2 {
+    double Pt, txx, tyy, txo, tyo, Gd, Mu, Rh, VyN, rg, Vx, txy, txyo;
+};
 
-    const double* __restrict__ Pt0 = in.p[PTS_PT];
-    const double* __restrict__ txx0 = in.p[PTS_TXX];
-    const double* __restrict__ tyy0 = in.p[PTS_TYY];
-    const double* __restrict__ txy0 = in.p[PTS_TXY];
-    const double* __restrict__ Vx0 = in.p[PTS_VX];
-    const double* __restrict__ Vy0 = in.p[PTS_VY];
-    const double* __restrict__ txxo = in.p[PTS_TXX_O];
-    const double* __restrict__ tyyo = in.p[PTS_TYY_O];
-    const double* __restrict__ txyo = in.p[PTS_TXY_O];
-    const double* __restrict__ Gdp = in.p[PTS_GDTAU];
-    const double* __restrict__ Mup = in.p[PTS_MUS];
-    const double* __restrict__ Rhp = in.p[PTS_DTAU_RHO];
-    const double* __restrict__ rgp = in.p[PTS_RHOG];
+template <bool VE, bool BF>
+__device__ __forceinline__ void load_row2(Row2& w, const Fields& in, int o, int sx, bool cin, bool fin, bool vtx) {
+    w = Row2{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (cin) {
+        w.Pt = in.p[PTS_PT][o];
+        w.txx = in.p[PTS_TXX][o];
+        w.tyy = in.p[PTS_TYY][o];
+        if constexpr (VE) {
+            w.txo = in.p[PTS_TXX_O][o];
+            w.tyo = in.p[PTS_TYY_O][o];
+        }
+        w.Gd = in.p[PTS_GDTAU][o];
+        w.Mu = in.p[PTS_MUS][o];
+        w.Rh = in.p[PTS_DTAU_RHO][o];
+        w.VyN = in.p[PTS_VY][o + sx];
+        if constexpr (BF) w.rg = in.p[PTS_RHOG][o];
+    }
+    if (fin) w.Vx = in.p[PTS_VX][o];
+    if (vtx) {
+        w.txy = in.p[PTS_TXY][o - 1 - sx];
+        if constexpr (VE) w.txyo = in.p[PTS_TXY_O][o - 1 - sx];
+    }
+}
+
+// Branch-free variant: every lane loads from an in-bounds (clamped) address,
+// so the number of loads in flight is static and the compiler can wait for
+// exactly the row it needs.  Lanes outside the grid then hold copies of edge
+// values; no stored output depends on them (each output's operands come from
+// lanes and rows inside its own validity range).
+template <bool VE, bool BF>
+__device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, int cx, int J, int nx, int sx) {
+    const int oc = min(max(cx, 0), nx - 1) + sx * J;
+    const int of = min(max(cx, 0), nx) + sx * J;
+    const int ov = min(max(cx - 1, 0), nx - 2) + sx * max(J - 1, 0);
+    w.Pt = in.p[PTS_PT][oc];
+    w.txx = in.p[PTS_TXX][oc];
+    w.tyy = in.p[PTS_TYY][oc];
+    w.txo = w.tyo = w.txyo = w.rg = 0.0;
+    if constexpr (VE) {
+        w.txo = in.p[PTS_TXX_O][oc];
+        w.tyo = in.p[PTS_TYY_O][oc];
+        w.txyo = in.p[PTS_TXY_O][ov];
+    }
+    w.Gd = in.p[PTS_GDTAU][oc];
+    w.Mu = in.p[PTS_MUS][oc];
+    w.Rh = in.p[PTS_DTAU_RHO][oc];
+    w.VyN = in.p[PTS_VY][oc + sx];
+    if constexpr (BF) w.rg = in.p[PTS_RHOG][oc];
+    w.Vx = in.p[PTS_VX][of];
+    w.txy = in.p[PTS_TXY][ov];
+}
+
+template <bool NT>
+__device__ __forceinline__ void st(double* p, double v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// ---------------------------------------------------------------------------
+// Single-pass iteration.  One 64-lane wave owns a strip of 62 output columns
+// (lane t handles column c = 62*strip - 1 + t; lanes 0 and 63 are halo) and a
+// chunk of rows, marching in y.  At row step J a lane computes
+//   * the new cell values Pt, τxx, τyy at (c, J)            (compute_Pt_τ!)
+//   * the new vertex τxy at (c-1, J-1)
+//   * the new Vx at face (c, J-1) and Vy at face (c, J)     (compute_V!)
+// with free-slip copies folded in (bc_y!(Vx), bc_x!(Vy)).  Neighbour values
+// across x come from DPP lane shifts; values from the previous row are kept
+// in registers.  The state is read from `in` and written to `out`, so every
+// array is streamed once per iteration.  Dirichlet (never-updated) entries of
+// Vx/Vy are not written: both ping-pong buffers hold them.  With PF the loads
+// of row J+1 are issued before row J is computed; with NT the stores bypass
+// the caches (the outputs are not read again in this launch).
+// ---------------------------------------------------------------------------
+template <bool VE, bool MUL, bool BF, bool PF, bool NT>
+__global__ __launch_bounds__(64) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
+                                              int nstrips, int ntiles) {
+    const int lane = threadIdx.x;
+    const int nx = in.nx, ny = in.ny, sx = in.sx;
     double* __restrict__ Pt1 = out.p[PTS_PT];
     double* __restrict__ txx1 = out.p[PTS_TXX];
     double* __restrict__ tyy1 = out.p[PTS_TYY];
     double* __restrict__ txy1 = out.p[PTS_TXY];
     double* __restrict__ Vx1 = out.p[PTS_VX];
     double* __restrict__ Vy1 = out.p[PTS_VY];
-
     const double dx = c.dx, dy = c.dy, _dx = c._dx, _dy = c._dy, r = c.r, Gdt = c.Gdt;
+    const int nb = gridDim.x;
 
-    // values carried from row J-1
-    double VxP = 0.0, GdP = 0.0, MuP = 0.0, GdPl = 0.0, MuPl = 0.0, RhP = 0.0, rgP = 0.0;
-    double Pt1P = 0.0, txx1P = 0.0, tyy1P = 0.0, txy1PP = 0.0;
-    const int j0 = max(jb - 1, 0), j1 = min(je, ny - 1);
-    double VyC = cin ? Vy0[cx + sx * j0] : 0.0;  // Vy0[c, J]
+    for (int b0 = blockIdx.x; b0 < ntiles; b0 += nb) {
+        int b = b0;
+        // XCD-aware order: the hardware deals consecutive blocks to the 8 XCDs
+        // round-robin; remap so each XCD gets a contiguous run of tiles and the
+        // halo columns shared by neighbouring strips stay in one L2.
+        if (nb >= ntiles && (ntiles & 7) == 0) b = (b & 7) * (ntiles >> 3) + (b >> 3);
+        const int strip = b % nstrips, chunk = b / nstrips;
+        const int cx = strip * 62 - 1 + lane;
+        const int jb = chunk * L;
+        const int je = min(jb + L, ny);
+        const bool own = lane >= 1 && lane <= 62;
+        const bool cin = cx >= 0 && cx < nx;       // cell column
+        const bool fin = cx >= 0 && cx <= nx;      // Vx face
+        const bool vin = cx >= 1 && cx <= nx - 1;  // vertex column cx-1 / interior Vx face
+        const bool vyin = cx >= 1 && cx <= nx - 2; // interior Vy column
 
-    for (int J = j0; J <= j1; ++J) {
-        const int o = cx + sx * J;
-        double Pt0v = 0.0, tx0 = 0.0, ty0 = 0.0, txo = 0.0, tyo = 0.0, Gd = 0.0, Mu = 0.0, Rh = 0.0, VyN = 0.0,
-               rg = 0.0, Vx0v = 0.0, tz0 = 0.0, tzo = 0.0;
-        if (cin) {
-            Pt0v = Pt0[o];
-            tx0 = txx0[o];
-            ty0 = tyy0[o];
-            if constexpr (VE) {
-                txo = txxo[o];
-                tyo = tyyo[o];
+        // values carried from row J-1
+        double VxP = 0.0, GdP = 0.0, MuP = 0.0, GdPl = 0.0, MuPl = 0.0, RhP = 0.0, rgP = 0.0;
+        double Pt1P = 0.0, txx1P = 0.0, tyy1P = 0.0, txy1PP = 0.0;
+        const int j0 = max(jb - 1, 0), j1 = min(je, ny - 1);
+        double VyC = in.p[PTS_VY][min(max(cx, 0), nx - 1) + sx * j0];  // Vy0[c, J]
+
+        auto step = [&](const int J, const Row2& cur) {
+            const int o = cx + sx * J;
+            const bool vtx = vin && J >= 1;
+            // lane exchanges (whole wave active)
+            const double VxR = from_right(cur.Vx);
+            const double GdL = from_left(cur.Gd);
+            const double MuL = from_left(cur.Mu);
+            const double VyCL = from_left(VyC);
+            const double txx1PL = from_left(txx1P);
+            const double Pt1PL = from_left(Pt1P);
+            const double RhPL = from_left(RhP);
+
+            // cell (c, J)
+            const double dVx = VxR - cur.Vx, dVy = cur.VyN - VyC;
+            const double ex = dfd<MUL>(dVx, dx, _dx), ey = dfd<MUL>(dVy, dy, _dy);
+            const double Pt1n = cur.Pt - r * cur.Gd * (ex + ey);
+            double txx1n, tyy1n;
+            if constexpr (!VE && !MUL) {
+                txx1n = tau_visc2d_div(cur.txx, cur.Gd, cur.Mu, dVx, dx);
+                tyy1n = tau_visc2d_div(cur.tyy, cur.Gd, cur.Mu, dVy, dy);
+            } else {
+                txx1n = tau_upd<VE>(cur.txx, cur.txo, cur.Gd, cur.Mu, Gdt, ex);
+                tyy1n = tau_upd<VE>(cur.tyy, cur.tyo, cur.Gd, cur.Mu, Gdt, ey);
             }
-            Gd = Gdp[o];
-            Mu = Mup[o];
-            Rh = Rhp[o];
-            VyN = Vy0[o + sx];
-            if constexpr (BF) rg = rgp[o];
-        }
-        if (fin) Vx0v = Vx0[o];
-        const bool vtx = vin && J >= 1;
-        if (vtx) {
-            tz0 = txy0[o - 1 - sx];
-            if constexpr (VE) tzo = txyo[o - 1 - sx];
-        }
-        // lane exchanges (whole wave active)
-        const double VxR = from_right(Vx0v);
-        const double GdL = from_left(Gd);
-        const double MuL = from_left(Mu);
-        const double VyCL = from_left(VyC);
-        const double txx1PL = from_left(txx1P);
-        const double Pt1PL = from_left(Pt1P);
-        const double RhPL = from_left(RhP);
+            // vertex (c-1, J-1): 4-cell means over [c-1..c] x [J-1..J]
+            const double ag = (GdPl + GdP + GdL + cur.Gd) * 0.25;
+            const double am = (MuPl + MuP + MuL + cur.Mu) * 0.25;
+            const double e = 0.5 * (dfd<MUL>(cur.Vx - VxP, dy, _dy) + dfd<MUL>(VyC - VyCL, dx, _dx));
+            const double txy1n = tau_upd<VE>(cur.txy, cur.txyo, ag, am, Gdt, e);
+            const double txy1nR = from_right(txy1n);
 
-        // cell (c, J)
-        const double dVx = VxR - Vx0v, dVy = VyN - VyC;
-        const double ex = dfd<MUL>(dVx, dx, _dx), ey = dfd<MUL>(dVy, dy, _dy);
-        const double Pt1n = Pt0v - r * Gd * (ex + ey);
-        double txx1n, tyy1n;
-        if constexpr (!VE && !MUL) {
-            txx1n = tau_visc2d_div(tx0, Gd, Mu, dVx, dx);
-            tyy1n = tau_visc2d_div(ty0, Gd, Mu, dVy, dy);
+            // stores of the new cell/vertex values owned by this tile
+            if (own && cin && J >= jb && J < je) {
+                st<NT>(Pt1 + o, Pt1n);
+                st<NT>(txx1 + o, txx1n);
+                st<NT>(tyy1 + o, tyy1n);
+            }
+            const int Jm = J - 1;
+            if (own && vtx && Jm >= jb && Jm < je) st<NT>(txy1 + o - 1 - sx, txy1n);
+
+            // Vx at face (c, J-1)
+            if (own && Jm >= jb && Jm < je && Jm >= 1 && Jm <= ny - 2) {
+                if (vin) {
+                    double R = dfd<MUL>(txx1P - txx1PL, dx, _dx) + dfd<MUL>(txy1n - txy1PP, dy, _dy) -
+                               dfd<MUL>(Pt1P - Pt1PL, dx, _dx);
+                    double v = VxP + R * (0.5 * (RhPL + RhP));
+                    st<NT>(Vx1 + o - sx, v);
+                    if (Jm == 1) Vx1[o - 2 * sx] = v;           // bc_y!: Vx[:,1] = Vx[:,2]
+                    if (Jm == ny - 2) Vx1[o] = v;               //        Vx[:,end] = Vx[:,end-1]
+                } else if (fin) {                               // Dirichlet faces c = 0, nx
+                    if (Jm == 1) Vx1[o - 2 * sx] = VxP;
+                    if (Jm == ny - 2) Vx1[o] = VxP;
+                }
+            }
+            // Vy at face (c, J)
+            if (own && J >= jb && J < je && J >= 1) {
+                if (vyin) {
+                    double R = dfd<MUL>(tyy1n - tyy1P, dy, _dy) + dfd<MUL>(txy1nR - txy1n, dx, _dx) -
+                               dfd<MUL>(Pt1n - Pt1P, dy, _dy);
+                    if constexpr (BF) R = R - (rgP + cur.rg) * 0.5;
+                    double v = VyC + R * (0.5 * (RhP + cur.Rh));
+                    st<NT>(Vy1 + o, v);
+                    if (cx == 1) Vy1[o - 1] = v;                // bc_x!: Vy[1,:] = Vy[2,:]
+                    if (cx == nx - 2) Vy1[o + 1] = v;           //        Vy[end,:] = Vy[end-1,:]
+                }
+            }
+            // Dirichlet rows of Vy (face rows 0 and ny): their x-copies
+            if (own && (cx == 1 || cx == nx - 2)) {
+                if (J == 0 && jb == 0) {
+                    if (cx == 1) Vy1[o - 1] = VyC;
+                    if (cx == nx - 2) Vy1[o + 1] = VyC;
+                }
+                if (J == ny - 1 && je == ny) {
+                    if (cx == 1) Vy1[o + sx - 1] = cur.VyN;
+                    if (cx == nx - 2) Vy1[o + sx + 1] = cur.VyN;
+                }
+            }
+            // carry
+            VxP = cur.Vx;
+            VyC = cur.VyN;
+            GdPl = GdL;
+            MuPl = MuL;
+            GdP = cur.Gd;
+            MuP = cur.Mu;
+            RhP = cur.Rh;
+            rgP = cur.rg;
+            Pt1P = Pt1n;
+            txx1P = txx1n;
+            tyy1P = tyy1n;
+            txy1PP = txy1n;
+        };
+        Row2 ra, rb;
+        if constexpr (PF) {
+            load_row2_clamped<VE, BF>(ra, in, cx, j0, nx, sx);
+            for (int J = j0; J <= j1; J += 2) {
+                if (J + 1 <= j1) load_row2_clamped<VE, BF>(rb, in, cx, J + 1, nx, sx);
+                step(J, ra);
+                if (J + 1 > j1) break;
+                if (J + 2 <= j1) load_row2_clamped<VE, BF>(ra, in, cx, J + 2, nx, sx);
+                step(J + 1, rb);
+            }
         } else {
-            txx1n = tau_upd<VE>(tx0, txo, Gd, Mu, Gdt, ex);
-            tyy1n = tau_upd<VE>(ty0, tyo, Gd, Mu, Gdt, ey);
-        }
-        // vertex (c-1, J-1): 4-cell means over [c-1..c] x [J-1..J]
-        const double ag = (GdPl + GdP + GdL + Gd) * 0.25;
-        const double am = (MuPl + MuP + MuL + Mu) * 0.25;
-        const double e = 0.5 * (dfd<MUL>(Vx0v - VxP, dy, _dy) + dfd<MUL>(VyC - VyCL, dx, _dx));
-        const double txy1n = tau_upd<VE>(tz0, tzo, ag, am, Gdt, e);
-        const double txy1nR = from_right(txy1n);
-
-        // stores of the new cell/vertex values owned by this tile
-        if (own && cin && J >= jb && J < je) {
-            Pt1[o] = Pt1n;
-            txx1[o] = txx1n;
-            tyy1[o] = tyy1n;
-        }
-        const int Jm = J - 1;
-        if (own && vtx && Jm >= jb && Jm < je) txy1[o - 1 - sx] = txy1n;
-
-        // Vx at face (c, J-1)
-        if (own && Jm >= jb && Jm < je && Jm >= 1 && Jm <= ny - 2) {
-            if (vin) {
-                double R = dfd<MUL>(txx1P - txx1PL, dx, _dx) + dfd<MUL>(txy1n - txy1PP, dy, _dy) -
-                           dfd<MUL>(Pt1P - Pt1PL, dx, _dx);
-                double v = VxP + R * (0.5 * (RhPL + RhP));
-                Vx1[o - sx] = v;
-                if (Jm == 1) Vx1[o - 2 * sx] = v;           // bc_y!: Vx[:,1] = Vx[:,2]
-                if (Jm == ny - 2) Vx1[o] = v;               //        Vx[:,end] = Vx[:,end-1]
-            } else if (fin) {                               // Dirichlet faces c = 0, nx
-                if (Jm == 1) Vx1[o - 2 * sx] = VxP;
-                if (Jm == ny - 2) Vx1[o] = VxP;
+            for (int J = j0; J <= j1; ++J) {
+                load_row2<VE, BF>(ra, in, cx + sx * J, sx, cin, fin, vin && J >= 1);
+                step(J, ra);
             }
         }
-        // Vy at face (c, J)
-        if (own && J >= jb && J < je && J >= 1) {
-            if (vyin) {
-                double R = dfd<MUL>(tyy1n - tyy1P, dy, _dy) + dfd<MUL>(txy1nR - txy1n, dx, _dx) -
-                           dfd<MUL>(Pt1n - Pt1P, dy, _dy);
-                if constexpr (BF) R = R - (rgP + rg) * 0.5;
-                double v = VyC + R * (0.5 * (RhP + Rh));
-                Vy1[o] = v;
-                if (cx == 1) Vy1[o - 1] = v;                // bc_x!: Vy[1,:] = Vy[2,:]
-                if (cx == nx - 2) Vy1[o + 1] = v;           //        Vy[end,:] = Vy[end-1,:]
-            }
-        }
-        // Dirichlet rows of Vy (face rows 0 and ny): their x-copies
-        if (own && (cx == 1 || cx == nx - 2)) {
-            if (J == 0 && jb == 0) {
-                if (cx == 1) Vy1[o - 1] = VyC;
-                if (cx == nx - 2) Vy1[o + 1] = VyC;
-            }
-            if (J == ny - 1 && je == ny) {
-                if (cx == 1) Vy1[o + sx - 1] = VyN;
-                if (cx == nx - 2) Vy1[o + sx + 1] = VyN;
-            }
-        }
-        // carry
-        VxP = Vx0v;
-        VyC = VyN;
-        GdPl = GdL;
-        MuPl = MuL;
-        GdP = Gd;
-        MuP = Mu;
-        RhP = Rh;
-        rgP = rg;
-        Pt1P = Pt1n;
-        txx1P = txx1n;
-        tyy1P = tyy1n;
-        txy1PP = txy1n;
     }
 }
 
@@ -378,24 +441,52 @@ hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
     return hipGetLastError();
 }
 
+// Resident single-wave blocks of one k2_iter instance on the whole device.
+template <bool VE, bool MUL, bool BF, bool PF, bool NT>
+static int concurrency2() {
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k2_iter<VE, MUL, BF, PF, NT>, 64, 0);
+        cached = std::max(1, cus) * std::max(1, per);
+    }
+    return cached;
+}
+
+template <bool VE, bool MUL, bool BF, bool PF, bool NT>
+static void go2v(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, bool persist) {
+    const int ns = (in.nx + 1 + 61) / 62;
+    const int conc = concurrency2<VE, MUL, BF, PF, NT>();
+    if (L < 1) {  // about one round of resident waves
+        const int nch = std::max(1, conc / ns);
+        L = (in.ny + nch - 1) / nch;
+    }
+    const int ntiles = ns * ((in.ny + L - 1) / L);
+    const int grid = persist ? std::min(ntiles, conc) : ntiles;
+    k2_iter<VE, MUL, BF, PF, NT><<<grid, 64, 0, s>>>(in, out, c, L, ns, ntiles);
+}
+
 template <bool VE, bool MUL, bool BF>
-static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, int ns,
-                int nblocks) {
-    k2_iter<VE, MUL, BF><<<nblocks, 64, 0, s>>>(in, out, c, L, ns);
+static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, int variant,
+                bool persist) {
+    switch (variant) {
+        case 0: go2v<VE, MUL, BF, false, false>(s, in, out, c, L, persist); break;
+        case 2: go2v<VE, MUL, BF, true, true>(s, in, out, c, L, persist); break;
+        case 3: go2v<VE, MUL, BF, false, true>(s, in, out, c, L, persist); break;
+        default: go2v<VE, MUL, BF, true, false>(s, in, out, c, L, persist); break;
+    }
 }
 
 hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
-                        bool mul, bool bf, int L) {
-    if (L < 1) L = 1;
-    const int ns = (in.nx + 1 + 61) / 62;
-    const int nc = (in.ny + L - 1) / L;
-    const int nb = ns * nc;
+                        bool mul, bool bf, int L, int variant, bool persist) {
     if (ve) {
-        if (mul) { if (bf) go2<true, true, true>(s, in, out, c, L, ns, nb); else go2<true, true, false>(s, in, out, c, L, ns, nb); }
-        else { if (bf) go2<true, false, true>(s, in, out, c, L, ns, nb); else go2<true, false, false>(s, in, out, c, L, ns, nb); }
+        if (mul) { if (bf) go2<true, true, true>(s, in, out, c, L, variant, persist); else go2<true, true, false>(s, in, out, c, L, variant, persist); }
+        else { if (bf) go2<true, false, true>(s, in, out, c, L, variant, persist); else go2<true, false, false>(s, in, out, c, L, variant, persist); }
     } else {
-        if (mul) { if (bf) go2<false, true, true>(s, in, out, c, L, ns, nb); else go2<false, true, false>(s, in, out, c, L, ns, nb); }
-        else { if (bf) go2<false, false, true>(s, in, out, c, L, ns, nb); else go2<false, false, false>(s, in, out, c, L, ns, nb); }
+        if (mul) { if (bf) go2<false, true, true>(s, in, out, c, L, variant, persist); else go2<false, true, false>(s, in, out, c, L, variant, persist); }
+        else { if (bf) go2<false, false, true>(s, in, out, c, L, variant, persist); else go2<false, false, false>(s, in, out, c, L, variant, persist); }
     }
     return hipGetLastError();
 }

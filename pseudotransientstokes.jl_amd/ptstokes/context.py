@@ -110,6 +110,11 @@ class Context:
         self._chk(self.lib.pts_init_inclusion(self.h, lx, ly, lz, mus0, musi, rad2, ebg, rhog_in, rhog_out,
                                               1 if igg else 0))
 
+    def init_grf_viscosity(self, modes, sigma, lo, hi, lx, ly, lz=10.0, musi=1e-3, rad2=1.0):
+        m = np.ascontiguousarray(np.asarray(modes, dtype=np.float64).reshape(-1, 4))
+        self._chk(self.lib.pts_init_grf_viscosity(self.h, m.shape[0], m.ctypes.data_as(_lib._dp), sigma, lo, hi,
+                                                  lx, ly, lz, musi, rad2))
+
     def smooth(self, dst, src, fact):
         self._chk(self.lib.pts_smooth(self.h, _fid(dst), _fid(src), float(fact)))
 
