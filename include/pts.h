@@ -188,6 +188,30 @@ typedef struct pts_solve_result {
  * when iter_max was reached (result still filled). */
 int pts_pt_solve(pts_ctx* ctx, const pts_pt_params* p, const pts_solve_params* sp, pts_solve_result* res);
 
+/* ---- mixed precision: FP32 PT sweeps inside FP64 defect correction ---------- */
+/* No counterpart in the reference (FP64 only).  Viscous grids, multiply form.
+ * Each outer step projects τ = 2μ ε̇(V), evaluates the FP64 residual (the stop
+ * test uses it), solves the correction with FP32 single-pass PT sweeps until
+ * its residual dropped by rtol_inner, and adds it to P and V in FP64. */
+typedef struct pts_dc_params {
+    double eps;            /* FP64 stop: err <= eps */
+    int32_t norm_kind;     /* 0: scaled RMS (reference err); 1: scaled max-norm */
+    int32_t max_outer;     /* at most this many corrections */
+    double rtol_inner;     /* inner stop: correction residual <= rtol_inner * initial */
+    int64_t nout_inner;    /* inner check interval */
+    int64_t iter_max_inner;
+} pts_dc_params;
+
+typedef struct pts_dc_result {
+    int32_t outer;         /* corrections applied */
+    int32_t converged;
+    int64_t inner_iters;   /* FP32 sweeps in total */
+    double err;            /* final FP64 err */
+    double err_outer[32];  /* FP64 err before each correction (first 32) */
+} pts_dc_result;
+
+int pts_dc_solve(pts_ctx* ctx, const pts_pt_params* p, const pts_dc_params* dp, pts_dc_result* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,8 @@ struct pts_ctx {
     int64_t nalloc = 0;  // elements per field allocation
     double* cur[PTS_NFIELDS] = {};
     double* alt[PTS_NFIELDS] = {};
+    float* cur32[PTS_NFIELDS] = {};  // FP32 correction state, constants and sources
+    float* alt32[PTS_NFIELDS] = {};
     bool alt_synced = false;
     hipStream_t stream = nullptr;
     double* red_partials = nullptr;
@@ -128,6 +130,7 @@ void touched(pts_ctx* c) { c->alt_synced = false; }
 
 int setdev(pts_ctx* c) {
     HIPCHK(c, hipSetDevice(c->g.device));
+    (void)hipGetLastError();  // launch checks below report only their own errors
     return PTS_OK;
 }
 
@@ -235,6 +238,8 @@ int pts_destroy(pts_ctx* c) {
     for (int q = 0; q < PTS_NFIELDS; ++q) {
         if (c->cur[q]) (void)hipFree(c->cur[q]);
         if (c->alt[q]) (void)hipFree(c->alt[q]);
+        if (c->cur32[q]) (void)hipFree(c->cur32[q]);
+        if (c->alt32[q] && c->alt32[q] != c->cur32[q]) (void)hipFree(c->alt32[q]);
     }
     if (c->red_partials) (void)hipFree(c->red_partials);
     if (c->red_out) (void)hipFree(c->red_out);
@@ -739,6 +744,132 @@ int pts_pt_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp,
     res->err = err;
     res->converged = err <= sp->eps ? 1 : 0;
     if (!res->converged) return fail(c, PTS_ERR_NOCONV, "iter_max reached with err > eps");
+    return PTS_OK;
+}
+
+// ---- mixed-precision defect correction ------------------------------------
+static int ensure32(pts_ctx* c, float** slot) {
+    if (*slot) return PTS_OK;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)c->nalloc * sizeof(float));
+    if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    HIPCHK(c, hipMemsetAsync(p, 0, (size_t)c->nalloc * sizeof(float), c->stream));
+    *slot = (float*)p;
+    c->bytes += c->nalloc * (int64_t)sizeof(float);
+    return PTS_OK;
+}
+
+static Fields32 fields32_of(const pts_ctx* c, float* const* src) {
+    Fields32 f;
+    for (int q = 0; q < PTS_NFIELDS; ++q) f.p[q] = src[q];
+    f.sx = c->sx;
+    f.sxy = c->sxy;
+    f.nx = c->nx;
+    f.ny = c->ny;
+    f.nz = c->nz;
+    return f;
+}
+
+int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pts_dc_result* res) {
+    if (!c || !p || !dp || !res || dp->nout_inner < 1 || dp->max_outer < 1) return fail(c, PTS_ERR_ARG, "bad argument");
+    if (c->g.ve) return fail(c, PTS_ERR_STATE, "defect correction is implemented for viscous grids");
+    if (int rc = setdev(c)) return rc;
+    if (int rc = need_iter_fields(c, false)) return rc;
+    std::memset(res, 0, sizeof(*res));
+    int st[16];
+    const int ns = state_fields(c, st);
+    for (int q = 0; q < ns; ++q) {
+        if (int rc = ensure32(c, &c->cur32[st[q]])) return rc;
+        if (int rc = ensure32(c, &c->alt32[st[q]])) return rc;
+    }
+    const int consts[] = {PTS_MUS, PTS_GDTAU, PTS_DTAU_RHO};
+    for (int f : consts) {
+        if (int rc = ensure32(c, &c->cur32[f])) return rc;
+        HIPCHK(c, launch_to_f32(c->stream, c->cur32[f], c->cur[f], c->nalloc));
+        c->alt32[f] = c->cur32[f];  // read-only: both sides of the ping-pong see it
+    }
+    const int srcs[] = {PTS_DIVV, PTS_RX, PTS_RY, PTS_RZ};
+    for (int f : srcs) {
+        if (f == PTS_RZ && c->nd == 2) continue;
+        if (int rc = ensure32(c, &c->cur32[f])) return rc;
+        c->alt32[f] = c->cur32[f];
+    }
+    const bool bf = c->g.body_force != 0;
+    const StepScalars s = scal(p->r, p->G, p->dt, p->dx, p->dy, c->nd == 3 ? p->dz : 1.0);
+    StepScalars32 s32;
+    s32.r = (float)s.r;
+    s32._dx = (float)s._dx;
+    s32._dy = (float)s._dy;
+    s32._dz = (float)s._dz;
+    for (int outer = 0;; ++outer) {
+        HIPCHK(c, launch_dc_project(c->stream, F(c), s, c->nd));
+        pts_norms nr;
+        HIPCHK(c, launch_norms_live(c->stream, F(c), F(c), s, c->nd, bf, true, c->red_partials, c->red_out));
+        HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, sizeof(double) * kRedVals, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        unpack_norms(c, c->h_red, &nr);
+        double err = pts_err_from_norms(&nr, c->nd, p->lx, dp->norm_kind, nullptr);
+        // a state with constant P or Vx (the zero initial guess) has no scale yet
+        if (!(nr.Pmax > nr.Pmin) || !(nr.Vmax > nr.Vmin)) {
+            bool finite = true;
+            for (int q = 0; q < 4; ++q) finite = finite && (nr.sumsq[q] == nr.sumsq[q]);
+            if (finite) err = INFINITY;
+        }
+        if (outer < 32) res->err_outer[outer] = err;
+        res->err = err;
+        if (err != err) return fail(c, PTS_ERR_NAN, "NaN in the FP64 residual");
+        if (err <= dp->eps) {
+            res->converged = 1;
+            break;
+        }
+        if (outer >= dp->max_outer) break;
+        // correction problem: sources, zero state, FP32 PT sweeps
+        HIPCHK(c, launch_dc_sources(c->stream, F(c), fields32_of(c, c->cur32), s, c->nd, bf));
+        for (int q = 0; q < ns; ++q) {
+            HIPCHK(c, hipMemsetAsync(c->cur32[st[q]], 0, (size_t)c->nalloc * sizeof(float), c->stream));
+            HIPCHK(c, hipMemsetAsync(c->alt32[st[q]], 0, (size_t)c->nalloc * sizeof(float), c->stream));
+        }
+        // Inner stop: the correction residual, weighted like err but with the
+        // scales of this outer step frozen (and guarded: P starts at zero),
+        // must drop by rtol_inner from its initial value (the outer residual).
+        const double dP = nr.Pmax - nr.Pmin, dV = nr.Vmax - nr.Vmin;
+        const double wR = p->lx / (dP > 0.0 ? dP : 1.0), wD = p->lx / (dV > 0.0 ? dV : 1.0);
+        auto weighted = [&](const double* ss, const double* ma) {
+            double e = 0.0;
+            for (int q = 0; q < 4; ++q) {
+                if (q == 2 && c->nd == 2) continue;
+                const double w = q == 3 ? wD : wR;
+                const double v = dp->norm_kind == 1 ? ma[q] * w : std::sqrt(ss[q]) * w / std::sqrt((double)nr.len[q]);
+                if (v != v) return v;
+                e = std::max(e, v);
+            }
+            return e;
+        };
+        const double err_in0 = weighted(nr.sumsq, nr.maxabs);
+        // the last correction only needs to bring err below eps (with margin)
+        const double rtol = std::isfinite(err) ? std::max(dp->rtol_inner, 0.25 * dp->eps / err) : dp->rtol_inner;
+        int64_t it = 0;
+        double err_in = err_in0;
+        while (err_in > rtol * err_in0 && it < dp->iter_max_inner) {
+            const int64_t n = std::min<int64_t>(dp->nout_inner, dp->iter_max_inner - it);
+            for (int64_t q = 0; q < n; ++q) {
+                HIPCHK(c, launch_dc_iter(c->stream, fields32_of(c, c->cur32), fields32_of(c, c->alt32), s32, c->nd));
+                for (int m = 0; m < ns; ++m) std::swap(c->cur32[st[m]], c->alt32[st[m]]);
+            }
+            it += n;
+            HIPCHK(c, launch_dc_norms(c->stream, fields32_of(c, c->cur32), s32, c->nd, c->red_partials, c->red_out));
+            HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            err_in = weighted(c->h_red, c->h_red + 4);
+            if (err_in != err_in) return fail(c, PTS_ERR_NAN, "NaN in the FP32 correction solve");
+        }
+        res->inner_iters += it;
+        HIPCHK(c, launch_dc_accumulate(c->stream, F(c), fields32_of(c, c->cur32), c->nd));
+        res->outer = outer + 1;
+    }
+    touched(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!res->converged) return fail(c, PTS_ERR_NOCONV, "defect correction did not reach eps");
     return PTS_OK;
 }
 

@@ -27,6 +27,13 @@ struct Fields {
     int nx, ny, nz;
 };
 
+// FP32 fields of the mixed-precision correction solve (same strides).
+struct Fields32 {
+    float* p[kNF];
+    int sx, sxy;
+    int nx, ny, nz;
+};
+
 // Reference shape of a field for the grid (shape[2] = 1 in 2D).
 void field_shape(int ndim, int nx, int ny, int nz, int field, int64_t out[3]);
 
@@ -81,7 +88,8 @@ __device__ __forceinline__ double tau_visc2d_div(double t, double g, double m, d
 // extent nb); a==1 also writes a=0, a==na-2 also writes a=na-1, same for b
 // (bc_y!/bc_z! etc., Stokes3D_ve_perf.jl:202-207; edge values end up equal to
 // the diagonal interior neighbour whatever the order of the copies).
-__device__ __forceinline__ void put_bc(double* V, int q, double v, int a, int na, int sa, int b, int nb, int sb) {
+template <typename T>
+__device__ __forceinline__ void put_bc(T* V, int q, T v, int a, int na, int sa, int b, int nb, int sb) {
     V[q] = v;
     const int da = (a == 1) ? -1 : ((a == na - 2) ? 1 : 0);
     const int db = (b == 1) ? -1 : ((b == nb - 2) ? 1 : 0);
@@ -94,6 +102,10 @@ __device__ __forceinline__ void put_bc(double* V, int q, double v, int a, int na
 struct StepScalars {
     double r, Gdt;
     double dx, dy, dz, _dx, _dy, _dz;
+};
+
+struct StepScalars32 {
+    float r, _dx, _dy, _dz;
 };
 
 // ---- kernel launchers (defined in the .hip files) ---------------------------
@@ -140,6 +152,17 @@ hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 // 3D single-pass iteration (ping-pong)
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
                         bool mul, bool bf, int planes_per_block, int tile_rows);
+
+// Mixed-precision defect correction (pts_dc.hip)
+hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim);
+hipError_t launch_dc_project(hipStream_t s, const Fields& f, const StepScalars& c, int ndim);
+hipError_t launch_dc_sources(hipStream_t s, const Fields& f, const Fields32& g, const StepScalars& c, int ndim,
+                             bool bf);
+hipError_t launch_dc_accumulate(hipStream_t s, const Fields& f, const Fields32& g, int ndim);
+hipError_t launch_to_f32(hipStream_t s, float* dst, const double* src, int64_t n);
+// out[0..3] = Σ R², out[4..7] = max|R| of the correction problem (Rx, Ry, Rz, ∇V)
+hipError_t launch_dc_norms(hipStream_t s, const Fields32& g, const StepScalars32& c, int ndim, double* partials,
+                           double* out);
 
 // Reductions.  `red` holds kRedSlots partial slots per block.
 constexpr int kRedBlocks = 512;

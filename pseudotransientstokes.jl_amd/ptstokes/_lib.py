@@ -70,6 +70,16 @@ class SolveResult(ctypes.Structure):
                 ("converged", ctypes.c_int32), ("nchecks", ctypes.c_int32)]
 
 
+class DcParams(ctypes.Structure):
+    _fields_ = [("eps", ctypes.c_double), ("norm_kind", ctypes.c_int32), ("max_outer", ctypes.c_int32),
+                ("rtol_inner", ctypes.c_double), ("nout_inner", ctypes.c_int64), ("iter_max_inner", ctypes.c_int64)]
+
+
+class DcResult(ctypes.Structure):
+    _fields_ = [("outer", ctypes.c_int32), ("converged", ctypes.c_int32), ("inner_iters", ctypes.c_int64),
+                ("err", ctypes.c_double), ("err_outer", ctypes.c_double * 32)]
+
+
 # (name, restype, argtypes) of every symbol in include/pts.h
 _vp = ctypes.c_void_p
 _d = ctypes.c_double
@@ -112,6 +122,7 @@ SIGNATURES = [
     ("pts_pt_iterate", _i, [_vp, ctypes.POINTER(PtParams), _i64]),
     ("pts_pt_solve", _i, [_vp, ctypes.POINTER(PtParams), ctypes.POINTER(SolveParams),
                           ctypes.POINTER(SolveResult)]),
+    ("pts_dc_solve", _i, [_vp, ctypes.POINTER(PtParams), ctypes.POINTER(DcParams), ctypes.POINTER(DcResult)]),
 ]
 
 _lib = None

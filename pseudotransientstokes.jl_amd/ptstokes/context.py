@@ -191,3 +191,16 @@ class Context:
             return out
         self._chk(rc)
         return out
+
+    def dc_solve(self, params, eps, norm_kind=1, max_outer=20, rtol_inner=1e-4, nout_inner=200,
+                 iter_max_inner=10 ** 7):
+        """FP32 PT correction sweeps inside FP64 defect correction (viscous grids)."""
+        dp = _lib.DcParams(eps, int(norm_kind), int(max_outer), rtol_inner, int(nout_inner), int(iter_max_inner))
+        res = _lib.DcResult()
+        rc = self.lib.pts_dc_solve(self.h, ctypes.byref(params), ctypes.byref(dp), ctypes.byref(res))
+        out = dict(outer=res.outer, converged=bool(res.converged), inner_iters=res.inner_iters, err=res.err,
+                   err_outer=list(res.err_outer)[: res.outer + 1])
+        if rc == _lib.PTS_ERR_NOCONV:
+            return out
+        self._chk(rc)
+        return out

@@ -16,7 +16,8 @@ import numpy as np
 from .context import Context
 
 _COMMON = dict(lx=10.0, ly=10.0, lz=10.0, mus0=1.0, musi=1e-3, rad2=1.0, ebg=1.0, eps=1e-8,
-               nsmooth=10, norm_kind=0, rhog_in=0.0, rhog_out=0.0, ic_igg=False)
+               nsmooth=10, norm_kind=0, rhog_in=0.0, rhog_out=0.0, ic_igg=False, grf_seed=None,
+               grf_sigma=1.0, grf_corr=0.5, grf_clip=2.0, grf_nmodes=1024)
 
 PRESETS = {
     # scripts/Stokes2D.jl:74-89
@@ -40,6 +41,20 @@ PRESETS = {
                                       nout=2000, CFL=0.8 / math.sqrt(3), Re=6.0 * math.pi, r=1.0, G=1.0,
                                       dt=1.0, n=63),
 }
+
+# Random-viscosity configurations (no reference script), viscous, max-norm
+# stop: log10 μ is a seeded Gaussian random field (σ = 1, correlation length
+# 0.5, clipped to ±2).  3D: plus the 1e-3 sphere carrying ρg_z = -1, εbg = 0;
+# the sphere inside the field reaches a 1e5 local contrast, for which the
+# reference's 10 smoothing passes leave the PT iteration unstable at fine
+# grids, so 20 passes are used.  2D (inversion): the field alone under pure
+# shear.
+PRESETS["Stokes3D_grf"] = dict(_COMMON, ndim=3, ve=False, fd="mul", nt=1, iter_max=10 ** 6, nout=500,
+                               CFL=0.8 / math.sqrt(3), Re=6.0 * math.pi, r=1.0, G=1.0, dt=1.0, n=63,
+                               ebg=0.0, rhog_in=-1.0, norm_kind=1, grf_seed=1234, nsmooth=20)
+PRESETS["Stokes2D_grf"] = dict(_COMMON, ndim=2, ve=False, fd="mul", nt=1, iter_max=10 ** 6, nout=500,
+                               CFL=0.9 / math.sqrt(2), Re=5 * math.pi, r=1.0, G=1.0, dt=1.0, n=127,
+                               norm_kind=1, grf_seed=1234, rad2=0.0)
 
 # 1D dispersion optimum (dispersion_analysis/dispersion_analysis_stokes1D.ipynb
 # cell 5 output): r = 1/4, Re = 9*sqrt(3)*pi/4
@@ -96,6 +111,11 @@ class StokesModel:
         p, c = self.p, self.ctx
         c.init_inclusion(p["lx"], p["ly"], p["lz"], p["mus0"], p["musi"], p["rad2"], p["ebg"], p["rhog_in"],
                          p["rhog_out"], p["ic_igg"])
+        if p["grf_seed"] is not None:
+            from . import grf
+            md = grf.modes(p["grf_seed"], p["ndim"], p["grf_corr"], p["grf_nmodes"])
+            c.init_grf_viscosity(md, p["grf_sigma"], -p["grf_clip"], p["grf_clip"], p["lx"], p["ly"], p["lz"],
+                                 p["musi"], p["rad2"])
         c.copy("Mus2", "Mus")
         for _ in range(p["nsmooth"]):
             c.smooth("Mus2", "Mus", 1.0)
@@ -153,6 +173,15 @@ class StokesModel:
         self.iters_per_step.append(res["iters"])
         self.ittot += res["iters"]
         self.err_history.append(res["err"])
+        return res
+
+    def solve_dc(self, rtol_inner=1e-4, nout_inner=200, max_outer=20):
+        """One steady solve by FP32 PT sweeps inside FP64 defect correction,
+        stopped on the same FP64 norm and eps as the FP64 PT loop."""
+        p = self.p
+        t0 = time.perf_counter()
+        res = self.ctx.dc_solve(self.pt_params(), p["eps"], p["norm_kind"], max_outer, rtol_inner, nout_inner)
+        self.timing["dc_s"] = time.perf_counter() - t0
         return res
 
     def run(self, use_graph=True):
