@@ -212,6 +212,33 @@ typedef struct pts_dc_result {
 
 int pts_dc_solve(pts_ctx* ctx, const pts_pt_params* p, const pts_dc_params* dp, pts_dc_result* res);
 
+/* ---- adjoint of the converged 2D viscous PT iteration --------------------
+ * No counterpart in the reference (the scripts have no adjoint).  Let Φ be one
+ * PT iteration (compute_Pt_τ!, compute_V!, bc_y!(Vx), bc_x!(Vy) of
+ * Stokes2D_ve_perf.jl:47-59,77-78 without the elastic terms), linear in the
+ * state (Pt, τxx, τyy, τxy, Vx, Vy) for fixed μ, Gdτ, dτ_Rho.  For an objective
+ * J(Vx, Vy) of the converged state the adjoint λ solves λ = Φᵀλ + ∂J/∂x; its
+ * iteration contracts like the forward one, so no iteration history is kept.
+ * Dirichlet (never-updated) velocity entries carry no adjoint.
+ * Requires a 2D viscous context whose Mus, Gdτ, dτ_Rho are set up and whose
+ * state is the converged forward solution (for pts_grad_logmu). */
+
+/* Set ∂J/∂Vx ((nx+1)*ny) and ∂J/∂Vy (nx*(ny+1)), host arrays, x fastest;
+ * NULL = zero; entries on Dirichlet faces are ignored.  Resets λ to zero. */
+int pts_adjoint_set_source(pts_ctx* ctx, const double* dJ_dVx, int64_t nvx, const double* dJ_dVy, int64_t nvy);
+/* λ of field PT, TXX, TYY, TXY, VX or VY (forward shapes), host arrays. */
+int pts_adjoint_upload(pts_ctx* ctx, int32_t field, const double* host, int64_t count);
+int pts_adjoint_download(pts_ctx* ctx, int32_t field, double* host, int64_t count);
+/* niter applications of λ <- Φᵀλ + ∂J/∂x (r, dx, dy of p are used). */
+int pts_adjoint_iterate(pts_ctx* ctx, const pts_pt_params* p, int64_t niter);
+/* Iterate until max|Δλ_V| <= eps * max|λ_V| over one iteration, checked every
+ * nout iterations (norm_kind and use_graph of sp are ignored).
+ * res->comp[0] = max|Δλ_V|, res->comp[1] = max|λ_V|. */
+int pts_adjoint_solve(pts_ctx* ctx, const pts_pt_params* p, const pts_solve_params* sp, pts_solve_result* res);
+/* ∂J/∂ln μ at the cell centres (μ = the Mus field), host array nx*ny, x
+ * fastest, from the current λ and the converged forward stresses. */
+int pts_grad_logmu(pts_ctx* ctx, const pts_pt_params* p, double* grad, int64_t count);
+
 #ifdef __cplusplus
 }
 #endif

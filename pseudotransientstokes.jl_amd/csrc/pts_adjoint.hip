@@ -1,0 +1,274 @@
+// pts_adjoint.hip -- adjoint of the converged 2D viscous PT iteration and the
+// gradient with respect to log viscosity (no counterpart in the reference).
+//
+// One forward PT iteration (multiply form, viscous) is the affine map
+//   Φ = B ∘ S_V ∘ S_Pτ :
+//   S_Pτ:  P' = P - r Gdτ ⊙ (D V),  τ' = a ⊙ τ + b ⊙ (E V),
+//          a = 1/(1 + Gdτ/μ), b = 2 Gdτ a (vertex means Ḡ, μ̄ for τxy)
+//   S_V :  V_int' = V_int + Rh ⊙ (Dτ τ' - ∇P' [- ρg])   (Rh: face mean of dτ_Rho)
+//   B   :  free-slip copies V[:,0] = V[:,1], ... (bc_y!(Vx), bc_x!(Vy))
+// D is the discrete divergence, E the strain rate (ε̇xx, ε̇yy, ε̇xy), Dτ the
+// stress divergence.  At the converged state x* = Φ(x*), the adjoint of
+// J(x*) is the fixed point of
+//   λ ← Φᵀ λ + ∂J/∂x = S_Pτᵀ S_Vᵀ Bᵀ λ + g,
+// which contracts at the rate of the forward iteration; no iteration history
+// is needed.  dJ/dm = (S_Vᵀ Bᵀ λ)_τ · ∂τ'/∂m.
+//
+// pass 1 (Bᵀ, S_Vᵀ): fold the boundary-copy adjoints onto their sources,
+//   w = Rh ⊙ λV_int,  λP += D(w)-like divergence,  λτ += -Dτᵀ w.
+// pass 2 (S_Pτᵀ, + g): λτ ← a ⊙ λτ,  λV ← λV - Dᵀ(r Gdτ ⊙ λP) + Eᵀ(b ⊙ λτ) + g,
+//   with λ = 0 on the Dirichlet (never-updated) velocity entries.
+// The adjoint fields use the field ids of the forward ones.
+#include <algorithm>
+
+#include "pts_internal.hpp"
+
+namespace pts {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+// adjoint weight w at a Vx face (I,J): Rh ⊙ folded λVx, zero off the interior
+__device__ __forceinline__ double wx_at(const Fields& f, const Fields& lam, int I, int J) {
+    const int nx = f.nx, ny = f.ny, sx = f.sx;
+    if (I < 1 || I > nx - 1 || J < 1 || J > ny - 2) return 0.0;
+    const int o = I + sx * J;
+    const double* L = lam.p[PTS_VX];
+    double l = L[o];
+    if (J == 1) l += L[o - sx];
+    if (J == ny - 2) l += L[o + sx];
+    const double* Rh = f.p[PTS_DTAU_RHO];
+    return l * (0.5 * (Rh[o - 1] + Rh[o]));
+}
+__device__ __forceinline__ double wy_at(const Fields& f, const Fields& lam, int I, int J) {
+    const int nx = f.nx, ny = f.ny, sx = f.sx;
+    if (I < 1 || I > nx - 2 || J < 1 || J > ny - 1) return 0.0;
+    const int o = I + sx * J;
+    const double* L = lam.p[PTS_VY];
+    double l = L[o];
+    if (I == 1) l += L[o - 1];
+    if (I == nx - 2) l += L[o + 1];
+    const double* Rh = f.p[PTS_DTAU_RHO];
+    return l * (0.5 * (Rh[o - sx] + Rh[o]));
+}
+
+// pass 1: in = λ, out = (λP', λτ', λV folded)
+__global__ __launch_bounds__(BX* BY) void k2_adj1(Fields f, Fields lam, Fields out, StepScalars c) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int nx = f.nx, ny = f.ny, sx = f.sx;
+    if (i > nx || j > ny) return;
+    const int o = i + sx * j;
+    if (i < nx && j < ny) {
+        const double wx0 = wx_at(f, lam, i, j), wx1 = wx_at(f, lam, i + 1, j);
+        const double wy0 = wy_at(f, lam, i, j), wy1 = wy_at(f, lam, i, j + 1);
+        out.p[PTS_PT][o] = lam.p[PTS_PT][o] + c._dx * (wx1 - wx0) + c._dy * (wy1 - wy0);
+        out.p[PTS_TXX][o] = lam.p[PTS_TXX][o] + c._dx * (wx0 - wx1);
+        out.p[PTS_TYY][o] = lam.p[PTS_TYY][o] + c._dy * (wy0 - wy1);
+    }
+    if (i < nx - 1 && j < ny - 1) {  // vertex (i,j): x-face i+1, y-face j+1
+        out.p[PTS_TXY][o] = lam.p[PTS_TXY][o] + c._dy * (wx_at(f, lam, i + 1, j) - wx_at(f, lam, i + 1, j + 1)) +
+                            c._dx * (wy_at(f, lam, i, j + 1) - wy_at(f, lam, i + 1, j + 1));
+    }
+    // folded velocity adjoints: sources get their copies, copies are zeroed
+    if (j < ny) {
+        const double* L = lam.p[PTS_VX];
+        double v = 0.0;
+        if (i >= 1 && i <= nx - 1 && j >= 1 && j <= ny - 2) {
+            v = L[o];
+            if (j == 1) v += L[o - sx];
+            if (j == ny - 2) v += L[o + sx];
+        }
+        out.p[PTS_VX][o] = v;
+    }
+    if (i < nx) {
+        const double* L = lam.p[PTS_VY];
+        double v = 0.0;
+        if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 1) {
+            v = L[o];
+            if (i == 1) v += L[o - 1];
+            if (i == nx - 2) v += L[o + 1];
+        }
+        out.p[PTS_VY][o] = v;
+    }
+}
+
+__device__ __forceinline__ double acell(const Fields& f, int o) {
+    const double g = f.p[PTS_GDTAU][o], m = f.p[PTS_MUS][o];
+    return 1.0 / (1.0 + g / m);
+}
+// b ⊙ λτ at cells (normal) and vertices (shear), zero off range
+__device__ __forceinline__ double pxx(const Fields& f, const Fields& l1, int i, int j) {
+    if (i < 0 || i >= f.nx || j < 0 || j >= f.ny) return 0.0;
+    const int o = i + f.sx * j;
+    return 2.0 * f.p[PTS_GDTAU][o] * acell(f, o) * l1.p[PTS_TXX][o];
+}
+__device__ __forceinline__ double pyy(const Fields& f, const Fields& l1, int i, int j) {
+    if (i < 0 || i >= f.nx || j < 0 || j >= f.ny) return 0.0;
+    const int o = i + f.sx * j;
+    return 2.0 * f.p[PTS_GDTAU][o] * acell(f, o) * l1.p[PTS_TYY][o];
+}
+__device__ __forceinline__ void vert_coef(const Fields& f, int o, double& a, double& b) {
+    const int sx = f.sx;
+    const double* Gd = f.p[PTS_GDTAU];
+    const double* Mu = f.p[PTS_MUS];
+    const double ag = 0.25 * (Gd[o] + Gd[o + 1] + Gd[o + sx] + Gd[o + sx + 1]);
+    const double am = 0.25 * (Mu[o] + Mu[o + 1] + Mu[o + sx] + Mu[o + sx + 1]);
+    a = 1.0 / (1.0 + ag / am);
+    b = 2.0 * ag * a;
+}
+__device__ __forceinline__ double pxy(const Fields& f, const Fields& l1, int i, int j) {
+    if (i < 0 || i >= f.nx - 1 || j < 0 || j >= f.ny - 1) return 0.0;
+    const int o = i + f.sx * j;
+    double a, b;
+    vert_coef(f, o, a, b);
+    return b * l1.p[PTS_TXY][o];
+}
+__device__ __forceinline__ double qP(const Fields& f, const Fields& l1, double r, int i, int j) {
+    if (i < 0 || i >= f.nx || j < 0 || j >= f.ny) return 0.0;
+    const int o = i + f.sx * j;
+    return r * f.p[PTS_GDTAU][o] * l1.p[PTS_PT][o];
+}
+
+// pass 2: in = pass-1 output, out = new λ
+__global__ __launch_bounds__(BX* BY) void k2_adj2(Fields f, Fields l1, Fields out, StepScalars c, const double* gx,
+                                                  const double* gy) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int nx = f.nx, ny = f.ny, sx = f.sx;
+    if (i > nx || j > ny) return;
+    const int o = i + sx * j;
+    if (i < nx && j < ny) {
+        out.p[PTS_PT][o] = l1.p[PTS_PT][o];
+        const double a = acell(f, o);
+        out.p[PTS_TXX][o] = a * l1.p[PTS_TXX][o];
+        out.p[PTS_TYY][o] = a * l1.p[PTS_TYY][o];
+    }
+    if (i < nx - 1 && j < ny - 1) {
+        double a, b;
+        vert_coef(f, o, a, b);
+        out.p[PTS_TXY][o] = a * l1.p[PTS_TXY][o];
+    }
+    if (j < ny) {  // Vx face (i, j); Dirichlet faces i = 0, nx carry no adjoint
+        double v = 0.0;
+        if (i >= 1 && i <= nx - 1) {
+            v = l1.p[PTS_VX][o] - c._dx * (qP(f, l1, c.r, i - 1, j) - qP(f, l1, c.r, i, j)) +
+                c._dx * (pxx(f, l1, i - 1, j) - pxx(f, l1, i, j)) +
+                0.5 * c._dy * (pxy(f, l1, i - 1, j - 1) - pxy(f, l1, i - 1, j));
+            if (gx) v += gx[o];
+        }
+        out.p[PTS_VX][o] = v;
+    }
+    if (i < nx) {  // Vy face (i, j); Dirichlet rows j = 0, ny
+        double v = 0.0;
+        if (j >= 1 && j <= ny - 1) {
+            v = l1.p[PTS_VY][o] - c._dy * (qP(f, l1, c.r, i, j - 1) - qP(f, l1, c.r, i, j)) +
+                c._dy * (pyy(f, l1, i, j - 1) - pyy(f, l1, i, j)) +
+                0.5 * c._dx * (pxy(f, l1, i - 1, j - 1) - pxy(f, l1, i, j - 1));
+            if (gy) v += gy[o];
+        }
+        out.p[PTS_VY][o] = v;
+    }
+}
+
+// ∂J/∂ln μ at cells from the pass-1 adjoint of the converged λ and the
+// forward stresses τ* (= τ' at the fixed point).
+__global__ __launch_bounds__(BX* BY) void k2_adj_grad(Fields f, Fields l1, double* grad) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int nx = f.nx, ny = f.ny, sx = f.sx;
+    if (i >= nx || j >= ny) return;
+    const int o = i + sx * j;
+    const double* Gd = f.p[PTS_GDTAU];
+    const double* Mu = f.p[PTS_MUS];
+    const double gm = Gd[o] / Mu[o];
+    double g = (l1.p[PTS_TXX][o] * f.p[PTS_TXX][o] + l1.p[PTS_TYY][o] * f.p[PTS_TYY][o]) * (gm / (1.0 + gm));
+    // shear: the (up to) four vertices around cell (i,j)
+    for (int dj = -1; dj <= 0; ++dj)
+        for (int di = -1; di <= 0; ++di) {
+            const int vi = i + di, vj = j + dj;
+            if (vi < 0 || vi >= nx - 1 || vj < 0 || vj >= ny - 1) continue;
+            const int v = vi + sx * vj;
+            const double ag = 0.25 * (Gd[v] + Gd[v + 1] + Gd[v + sx] + Gd[v + sx + 1]);
+            const double am = 0.25 * (Mu[v] + Mu[v + 1] + Mu[v + sx] + Mu[v + sx + 1]);
+            g += l1.p[PTS_TXY][v] * f.p[PTS_TXY][v] * 0.25 * Mu[o] * (ag / (am * am)) / (1.0 + ag / am);
+        }
+    grad[i + (int64_t)nx * j] = g;
+}
+
+// NaN-propagating max (fmax drops a NaN operand)
+__device__ __forceinline__ double nmax(double a, double b) { return (a != a) ? a : ((b != b) ? b : fmax(a, b)); }
+
+// max |λV - λV_prev| and max |λV| over both velocity components
+__global__ __launch_bounds__(256) void k2_adj_change(Fields f, Fields a, Fields b, double* part) {
+    __shared__ double s0[256], s1[256];
+    const int nx = f.nx, ny = f.ny, sx = f.sx;
+    const int64_t tot = (int64_t)(nx + 1) * (ny + 1);
+    double d = 0.0, m = 0.0;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % (nx + 1)), j = (int)(q / (nx + 1));
+        const int o = i + sx * j;
+        if (j < ny) {
+            d = nmax(d, fabs(a.p[PTS_VX][o] - b.p[PTS_VX][o]));
+            m = nmax(m, fabs(a.p[PTS_VX][o]));
+        }
+        if (i < nx) {
+            d = nmax(d, fabs(a.p[PTS_VY][o] - b.p[PTS_VY][o]));
+            m = nmax(m, fabs(a.p[PTS_VY][o]));
+        }
+    }
+    s0[threadIdx.x] = d;
+    s1[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s0[threadIdx.x] = nmax(s0[threadIdx.x], s0[threadIdx.x + w]);
+            s1[threadIdx.x] = nmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+__global__ void k_max2_final(const double* part, int nb, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double d = 0.0, m = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        d = nmax(d, part[2 * b]);
+        m = nmax(m, part[2 * b + 1]);
+    }
+    out[0] = d;
+    out[1] = m;
+}
+
+inline dim3 grid2(int n1, int n2) { return dim3((n1 + BX - 1) / BX, (n2 + BY - 1) / BY, 1); }
+
+}  // namespace
+
+hipError_t launch2_adj_step(hipStream_t s, const Fields& f, const Fields& lam, const Fields& tmp, const Fields& out,
+                            const StepScalars& c, const double* gx, const double* gy) {
+    k2_adj1<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, lam, tmp, c);
+    k2_adj2<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, tmp, out, c, gx, gy);
+    return hipGetLastError();
+}
+
+hipError_t launch2_adj_pass1(hipStream_t s, const Fields& f, const Fields& lam, const Fields& tmp,
+                             const StepScalars& c) {
+    k2_adj1<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, lam, tmp, c);
+    return hipGetLastError();
+}
+
+hipError_t launch2_adj_grad(hipStream_t s, const Fields& f, const Fields& l1, double* grad) {
+    k2_adj_grad<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s>>>(f, l1, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch2_adj_change(hipStream_t s, const Fields& f, const Fields& a, const Fields& b, double* part,
+                              double* out) {
+    k2_adj_change<<<kRedBlocks, 256, 0, s>>>(f, a, b, part);
+    k_max2_final<<<1, 64, 0, s>>>(part, kRedBlocks, out);
+    return hipGetLastError();
+}
+
+}  // namespace pts

@@ -39,12 +39,19 @@ struct pts_ctx {
     double* red_out = nullptr;
     double* h_red = nullptr;
     int64_t bytes = 0;
-    int rows_per_block = 128;   // 2D single-pass: rows per block (0: one round of resident waves)
+    int rows_per_block = -128;  // 2D single-pass: rows per block (0: one round of resident waves; <0: that, capped)
     int variant_2d = 1;         // 2D single-pass code variant (prefetch)
     bool persist_2d = false;    // 2D single-pass: persistent waves looping over tiles
-    int planes_per_block = 64;  // 3D single-pass: z planes per block
+    int planes_per_block = -64; // 3D single-pass: z planes per block (<0: one round of resident blocks, capped)
     int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
+    // adjoint state λ (ping-pong by field id), the pass-1 scratch, the
+    // source ∂J/∂Vx, ∂J/∂Vy and the gradient buffer
+    double* adj[2][PTS_NFIELDS] = {};
+    double* adj_tmp[PTS_NFIELDS] = {};
+    double* adj_src[2] = {};
+    double* adj_grad = nullptr;
+    int adj_cur = 0;
     std::string err;
     // graphs of n single-pass iterations keyed by (n, parity of the start buffer)
     std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
@@ -210,10 +217,10 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     }
     c->sxy = (int)sxy;
     c->nalloc = na;
-    if (const char* e = std::getenv("PTS_ROWS_PER_BLOCK")) c->rows_per_block = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("PTS_ROWS_PER_BLOCK")) c->rows_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_2D_VARIANT")) c->variant_2d = std::atoi(e);
     if (const char* e = std::getenv("PTS_2D_PERSIST")) c->persist_2d = std::atoi(e) != 0;
-    if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
     hipError_t e = hipSetDevice(g->device);
@@ -240,7 +247,13 @@ int pts_destroy(pts_ctx* c) {
         if (c->alt[q]) (void)hipFree(c->alt[q]);
         if (c->cur32[q]) (void)hipFree(c->cur32[q]);
         if (c->alt32[q] && c->alt32[q] != c->cur32[q]) (void)hipFree(c->alt32[q]);
+        if (c->adj[0][q]) (void)hipFree(c->adj[0][q]);
+        if (c->adj[1][q]) (void)hipFree(c->adj[1][q]);
+        if (c->adj_tmp[q]) (void)hipFree(c->adj_tmp[q]);
     }
+    for (double* p : c->adj_src)
+        if (p) (void)hipFree(p);
+    if (c->adj_grad) (void)hipFree(c->adj_grad);
     if (c->red_partials) (void)hipFree(c->red_partials);
     if (c->red_out) (void)hipFree(c->red_out);
     if (c->h_red) (void)hipHostFree(c->h_red);
@@ -270,10 +283,8 @@ int pts_device_bytes(const pts_ctx* c, int64_t* bytes) {
     return PTS_OK;
 }
 
-static int copy_field(pts_ctx* c, int f, double* host, const double* chost, int64_t count, bool up) {
-    if (!c || (up ? !chost : !host)) return fail(c, PTS_ERR_ARG, "null argument");
-    if (int rc = setdev(c)) return rc;
-    if (int rc = ensure(c, f)) return rc;
+// Dense host array (x fastest) <-> the strided device layout of field f at `base`.
+static int copy_dense(pts_ctx* c, int f, double* base, double* host, const double* chost, int64_t count, bool up) {
     int64_t s[3];
     field_shape(c->nd, c->nx, c->ny, c->nz, f, s);
     if (count != s[0] * s[1] * s[2])
@@ -281,7 +292,7 @@ static int copy_field(pts_ctx* c, int f, double* host, const double* chost, int6
     if (count == 0) return PTS_OK;
     const size_t w = (size_t)s[0] * sizeof(double);
     for (int64_t k = 0; k < s[2]; ++k) {
-        double* dev = c->cur[f] + k * (int64_t)c->sxy;
+        double* dev = base + k * (int64_t)c->sxy;
         if (up)
             HIPCHK(c, hipMemcpy2DAsync(dev, (size_t)c->sx * sizeof(double), chost + k * s[0] * s[1], w, w, (size_t)s[1],
                                        hipMemcpyHostToDevice, c->stream));
@@ -290,6 +301,14 @@ static int copy_field(pts_ctx* c, int f, double* host, const double* chost, int6
                                        hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PTS_OK;
+}
+
+static int copy_field(pts_ctx* c, int f, double* host, const double* chost, int64_t count, bool up) {
+    if (!c || (up ? !chost : !host)) return fail(c, PTS_ERR_ARG, "null argument");
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure(c, f)) return rc;
+    if (int rc = copy_dense(c, f, c->cur[f], host, chost, count, up)) return rc;
     if (up) touched(c);
     return PTS_OK;
 }
@@ -870,6 +889,151 @@ int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pt
     touched(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (!res->converged) return fail(c, PTS_ERR_NOCONV, "defect correction did not reach eps");
+    return PTS_OK;
+}
+
+
+// ---- adjoint of the converged 2D viscous PT iteration ----------------------
+static const int kAdj2[] = {PTS_PT, PTS_TXX, PTS_TYY, PTS_TXY, PTS_VX, PTS_VY};
+
+static int alloc_zero(pts_ctx* c, double** slot) {
+    if (*slot) return PTS_OK;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)c->nalloc * sizeof(double));
+    if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    HIPCHK(c, hipMemsetAsync(p, 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    *slot = (double*)p;
+    c->bytes += c->nalloc * (int64_t)sizeof(double);
+    return PTS_OK;
+}
+
+static bool adj_field(int f) {
+    for (int q : kAdj2)
+        if (q == f) return true;
+    return false;
+}
+
+static int prepare_adj(pts_ctx* c) {
+    if (c->nd != 2 || c->g.ve) return fail(c, PTS_ERR_STATE, "the adjoint is implemented for 2D viscous grids");
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure_list(c, {PTS_PT, PTS_VX, PTS_VY, PTS_TXX, PTS_TYY, PTS_TXY, PTS_MUS, PTS_GDTAU, PTS_DTAU_RHO}))
+        return rc;
+    for (int f : kAdj2) {
+        if (int rc = alloc_zero(c, &c->adj[0][f])) return rc;
+        if (int rc = alloc_zero(c, &c->adj[1][f])) return rc;
+        if (int rc = alloc_zero(c, &c->adj_tmp[f])) return rc;
+    }
+    for (double*& p : c->adj_src)
+        if (int rc = alloc_zero(c, &p)) return rc;
+    return alloc_zero(c, &c->adj_grad);
+}
+
+static Fields adj_fields(const pts_ctx* c, double* const* src) {
+    Fields f = F(c);
+    for (int q = 0; q < PTS_NFIELDS; ++q) f.p[q] = nullptr;
+    for (int q : kAdj2) f.p[q] = src[q];
+    return f;
+}
+
+static StepScalars adj_scal(const pts_ctx* c, const pts_pt_params* p) {
+    return scal(p->r, p->G, p->dt, p->dx, p->dy, 1.0);
+}
+
+int pts_adjoint_set_source(pts_ctx* c, const double* dJ_dVx, int64_t nvx, const double* dJ_dVy, int64_t nvy) {
+    if (!c) return fail(c, PTS_ERR_ARG, "null argument");
+    if (int rc = prepare_adj(c)) return rc;
+    const int fs[2] = {PTS_VX, PTS_VY};
+    const double* hs[2] = {dJ_dVx, dJ_dVy};
+    const int64_t ns[2] = {nvx, nvy};
+    for (int q = 0; q < 2; ++q) {
+        HIPCHK(c, hipMemsetAsync(c->adj_src[q], 0, (size_t)c->nalloc * sizeof(double), c->stream));
+        if (hs[q])
+            if (int rc = copy_dense(c, fs[q], c->adj_src[q], nullptr, hs[q], ns[q], true)) return rc;
+    }
+    for (int f : kAdj2)
+        for (int b = 0; b < 2; ++b)
+            HIPCHK(c, hipMemsetAsync(c->adj[b][f], 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PTS_OK;
+}
+
+int pts_adjoint_upload(pts_ctx* c, int f, const double* host, int64_t count) {
+    if (!c || !host) return fail(c, PTS_ERR_ARG, "null argument");
+    if (!adj_field(f)) return fail(c, PTS_ERR_ARG, "field has no adjoint");
+    if (int rc = prepare_adj(c)) return rc;
+    return copy_dense(c, f, c->adj[c->adj_cur][f], nullptr, host, count, true);
+}
+
+int pts_adjoint_download(pts_ctx* c, int f, double* host, int64_t count) {
+    if (!c || !host) return fail(c, PTS_ERR_ARG, "null argument");
+    if (!adj_field(f)) return fail(c, PTS_ERR_ARG, "field has no adjoint");
+    if (int rc = prepare_adj(c)) return rc;
+    return copy_dense(c, f, c->adj[c->adj_cur][f], host, nullptr, count, false);
+}
+
+static int adj_steps(pts_ctx* c, const StepScalars& s, int64_t n) {
+    const Fields f = F(c), t = adj_fields(c, c->adj_tmp);
+    for (int64_t q = 0; q < n; ++q) {
+        const int a = c->adj_cur;
+        HIPCHK(c, launch2_adj_step(c->stream, f, adj_fields(c, c->adj[a]), t, adj_fields(c, c->adj[a ^ 1]), s,
+                                   c->adj_src[0], c->adj_src[1]));
+        c->adj_cur = a ^ 1;
+    }
+    return PTS_OK;
+}
+
+int pts_adjoint_iterate(pts_ctx* c, const pts_pt_params* p, int64_t niter) {
+    if (!c || !p || niter < 0) return fail(c, PTS_ERR_ARG, "bad argument");
+    if (int rc = prepare_adj(c)) return rc;
+    if (int rc = adj_steps(c, adj_scal(c, p), niter)) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PTS_OK;
+}
+
+int pts_adjoint_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp, pts_solve_result* res) {
+    if (!c || !p || !sp || !res || sp->nout < 1) return fail(c, PTS_ERR_ARG, "bad argument");
+    if (int rc = prepare_adj(c)) return rc;
+    const StepScalars s = adj_scal(c, p);
+    std::memset(res, 0, sizeof(*res));
+    int64_t iter = 0;
+    double err = INFINITY;
+    while (err > sp->eps && iter < sp->iter_max) {
+        const int64_t n = std::min<int64_t>(sp->nout, sp->iter_max - iter);
+        if (int rc = adj_steps(c, s, n)) return rc;
+        iter += n;
+        // change of λV over the last iteration relative to max |λV|
+        HIPCHK(c, launch2_adj_change(c->stream, F(c), adj_fields(c, c->adj[c->adj_cur]),
+                                     adj_fields(c, c->adj[c->adj_cur ^ 1]), c->red_partials, c->red_out));
+        HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const double d = c->h_red[0], m = c->h_red[1];
+        res->comp[0] = d;
+        res->comp[1] = m;
+        res->nchecks += 1;
+        if (d != d || m != m) {
+            res->iters = iter;
+            res->err = NAN;
+            return fail(c, PTS_ERR_NAN, "NaN in the adjoint state after " + std::to_string(iter) + " iterations");
+        }
+        err = m > 0.0 ? d / m : 0.0;
+    }
+    res->iters = iter;
+    res->err = err;
+    res->converged = err <= sp->eps ? 1 : 0;
+    if (!res->converged) return fail(c, PTS_ERR_NOCONV, "adjoint: iter_max reached with err > eps");
+    return PTS_OK;
+}
+
+int pts_grad_logmu(pts_ctx* c, const pts_pt_params* p, double* grad, int64_t count) {
+    if (!c || !p || !grad) return fail(c, PTS_ERR_ARG, "null argument");
+    if (int rc = prepare_adj(c)) return rc;
+    if (count != (int64_t)c->nx * c->ny)
+        return fail(c, PTS_ERR_ARG, "count " + std::to_string(count) + " != nx*ny");
+    const Fields t = adj_fields(c, c->adj_tmp);
+    HIPCHK(c, launch2_adj_pass1(c->stream, F(c), adj_fields(c, c->adj[c->adj_cur]), t, adj_scal(c, p)));
+    HIPCHK(c, launch2_adj_grad(c->stream, F(c), t, c->adj_grad));
+    HIPCHK(c, hipMemcpyAsync(grad, c->adj_grad, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return PTS_OK;
 }
 

@@ -164,6 +164,16 @@ hipError_t launch_to_f32(hipStream_t s, float* dst, const double* src, int64_t n
 hipError_t launch_dc_norms(hipStream_t s, const Fields32& g, const StepScalars32& c, int ndim, double* partials,
                            double* out);
 
+// Adjoint of the converged 2D viscous iteration.  `lam`, `tmp`, `out` carry
+// the adjoint fields under the forward field ids (PT, TXX, TYY, TXY, VX, VY).
+hipError_t launch2_adj_step(hipStream_t s, const Fields& f, const Fields& lam, const Fields& tmp, const Fields& out,
+                            const StepScalars& c, const double* gx, const double* gy);
+hipError_t launch2_adj_pass1(hipStream_t s, const Fields& f, const Fields& lam, const Fields& tmp,
+                             const StepScalars& c);
+hipError_t launch2_adj_grad(hipStream_t s, const Fields& f, const Fields& l1, double* grad);
+hipError_t launch2_adj_change(hipStream_t s, const Fields& f, const Fields& a, const Fields& b, double* part,
+                              double* out);
+
 // Reductions.  `red` holds kRedSlots partial slots per block.
 constexpr int kRedBlocks = 512;
 constexpr int kRedVals = 12;  // Vmin Vmax Pmin Pmax ss[4] ma[4]

@@ -286,14 +286,36 @@ __global__ __launch_bounds__(64 * BY) void k3_iter(const Fields in, const Fields
     }
 }
 
+// Resident blocks of one k3_iter instance on the whole device.
+template <bool VE, bool MUL, bool BF, int BY>
+int concurrency3() {
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k3_iter<VE, MUL, BF, BY>, 64 * BY, 0);
+        cached = std::max(1, cus) * std::max(1, per);
+    }
+    return cached;
+}
+
+// Lz planes per block; Lz < 0: about one round of resident blocks, within
+// [4, -Lz] planes (small grids get short columns, large ones long marches).
 template <bool VE, bool MUL, bool BF>
 void go3(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int Lz, int by) {
     const int ntx = (in.nx + 1 + 61) / 62;
-    const int nch = (in.nz + Lz - 1) / Lz;
     auto launch = [&](auto byc) {
         constexpr int BY = decltype(byc)::value;
         const int nty = (in.ny + 1 + (BY - 2) - 1) / (BY - 2);
-        k3_iter<VE, MUL, BF, BY><<<ntx * nty * nch, dim3(64, BY), 0, s>>>(in, out, c, Lz, ntx, nty);
+        int L = Lz;
+        if (L < 0) {
+            const int nch = std::max(1, concurrency3<VE, MUL, BF, BY>() / (ntx * nty));
+            L = std::max(4, std::min((in.nz + nch - 1) / nch, -Lz));
+        }
+        L = std::max(1, L);
+        const int nch = (in.nz + L - 1) / L;
+        k3_iter<VE, MUL, BF, BY><<<ntx * nty * nch, dim3(64, BY), 0, s>>>(in, out, c, L, ntx, nty);
     };
     if (by == 8) launch(std::integral_constant<int, 8>{});
     else if (by == 16) launch(std::integral_constant<int, 16>{});
@@ -304,7 +326,7 @@ void go3(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& 
 
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve, bool mul,
                         bool bf, int Lz, int by) {
-    if (Lz < 1) Lz = 1;
+    if (Lz == 0) Lz = 1;
     if (ve) {
         if (mul) { if (bf) go3<true, true, true>(s, in, out, c, Lz, by); else go3<true, true, false>(s, in, out, c, Lz, by); }
         else { if (bf) go3<true, false, true>(s, in, out, c, Lz, by); else go3<true, false, false>(s, in, out, c, Lz, by); }

@@ -459,9 +459,11 @@ template <bool VE, bool MUL, bool BF, bool PF, bool NT>
 static void go2v(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, bool persist) {
     const int ns = (in.nx + 1 + 61) / 62;
     const int conc = concurrency2<VE, MUL, BF, PF, NT>();
-    if (L < 1) {  // about one round of resident waves
+    if (L < 1) {  // about one round of resident waves (L < 0: within [4, -L] rows)
         const int nch = std::max(1, conc / ns);
+        const int cap = -L;
         L = (in.ny + nch - 1) / nch;
+        if (cap > 0) L = std::max(4, std::min(L, cap));
     }
     const int ntiles = ns * ((in.ny + L - 1) / L);
     const int grid = persist ? std::min(ntiles, conc) : ntiles;

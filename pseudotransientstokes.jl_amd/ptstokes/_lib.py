@@ -123,6 +123,13 @@ SIGNATURES = [
     ("pts_pt_solve", _i, [_vp, ctypes.POINTER(PtParams), ctypes.POINTER(SolveParams),
                           ctypes.POINTER(SolveResult)]),
     ("pts_dc_solve", _i, [_vp, ctypes.POINTER(PtParams), ctypes.POINTER(DcParams), ctypes.POINTER(DcResult)]),
+    ("pts_adjoint_set_source", _i, [_vp, _dp, _i64, _dp, _i64]),
+    ("pts_adjoint_upload", _i, [_vp, _i32, _dp, _i64]),
+    ("pts_adjoint_download", _i, [_vp, _i32, _dp, _i64]),
+    ("pts_adjoint_iterate", _i, [_vp, ctypes.POINTER(PtParams), _i64]),
+    ("pts_adjoint_solve", _i, [_vp, ctypes.POINTER(PtParams), ctypes.POINTER(SolveParams),
+                               ctypes.POINTER(SolveResult)]),
+    ("pts_grad_logmu", _i, [_vp, ctypes.POINTER(PtParams), _dp, _i64]),
 ]
 
 _lib = None

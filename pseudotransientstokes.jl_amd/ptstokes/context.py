@@ -204,3 +204,50 @@ class Context:
             return out
         self._chk(rc)
         return out
+
+    # -- adjoint of the converged 2D viscous iteration ------------------------
+    def _dense(self, name, arr):
+        shp = self.shape(name)
+        a = np.asarray(arr, dtype=np.float64)
+        if a.shape != shp:
+            raise ValueError(f"{name}: shape {a.shape} != {shp}")
+        return np.ascontiguousarray(a.reshape(-1, order="F"))
+
+    def adjoint_set_source(self, dJ_dVx=None, dJ_dVy=None):
+        """Set ∂J/∂Vx, ∂J/∂Vy (forward shapes, None = zero) and reset λ."""
+        gx = None if dJ_dVx is None else self._dense("Vx", dJ_dVx)
+        gy = None if dJ_dVy is None else self._dense("Vy", dJ_dVy)
+        px = gx.ctypes.data_as(_lib._dp) if gx is not None else None
+        py = gy.ctypes.data_as(_lib._dp) if gy is not None else None
+        self._chk(self.lib.pts_adjoint_set_source(self.h, px, 0 if gx is None else gx.size, py,
+                                                  0 if gy is None else gy.size))
+
+    def adjoint_upload(self, name, arr):
+        flat = self._dense(name, arr)
+        self._chk(self.lib.pts_adjoint_upload(self.h, _fid(name), flat.ctypes.data_as(_lib._dp), flat.size))
+
+    def adjoint_download(self, name):
+        shp = self.shape(name)
+        out = np.empty(int(np.prod(shp)), dtype=np.float64)
+        self._chk(self.lib.pts_adjoint_download(self.h, _fid(name), out.ctypes.data_as(_lib._dp), out.size))
+        return out.reshape(shp, order="F")
+
+    def adjoint_iterate(self, params, niter):
+        self._chk(self.lib.pts_adjoint_iterate(self.h, ctypes.byref(params), int(niter)))
+
+    def adjoint_solve(self, params, eps, nout, iter_max):
+        sp = _lib.SolveParams(eps, int(nout), int(iter_max), 0, 0)
+        res = _lib.SolveResult()
+        rc = self.lib.pts_adjoint_solve(self.h, ctypes.byref(params), ctypes.byref(sp), ctypes.byref(res))
+        out = dict(iters=res.iters, err=res.err, change=res.comp[0], lam_max=res.comp[1],
+                   converged=bool(res.converged), nchecks=res.nchecks)
+        if rc == _lib.PTS_ERR_NOCONV:
+            return out
+        self._chk(rc)
+        return out
+
+    def grad_logmu(self, params):
+        """∂J/∂ln μ on the cell centres, shape (nx, ny)."""
+        out = np.empty(self.nx * self.ny, dtype=np.float64)
+        self._chk(self.lib.pts_grad_logmu(self.h, ctypes.byref(params), out.ctypes.data_as(_lib._dp), out.size))
+        return out.reshape((self.nx, self.ny), order="F")

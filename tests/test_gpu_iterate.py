@@ -85,6 +85,10 @@ VARIANTS = [
     ("Stokes3D_ve_multixpu", dict(n=16, rhog_in=-1.0), {"PTS_PLANES_PER_BLOCK": "1"}),
     ("Stokes2D_ve_perf", dict(nx=90, ny=41), {"PTS_ROWS_PER_BLOCK": "5"}),
     ("Stokes2D", dict(nx=64, ny=33), {"PTS_ROWS_PER_BLOCK": "1"}),
+    ("Stokes2D_ve_perf", dict(nx=300, ny=290), {"PTS_ROWS_PER_BLOCK": "128"}),
+    ("Stokes2D_ve_perf", dict(nx=130, ny=70), {"PTS_ROWS_PER_BLOCK": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=80), {"PTS_PLANES_PER_BLOCK": "64"}),
+    ("Stokes3D_ve_perf", dict(nx=40, ny=30, nz=50), {"PTS_PLANES_PER_BLOCK": "-16"}),
 ]
 
 
