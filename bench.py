@@ -103,6 +103,45 @@ def bench_3d(n, steps, warmup, ev):
     return out
 
 
+def bench_tts(n, nout=500, rtol_inner=1e-4):
+    """Time to ‖res‖∞ < 1e-8 on the 3D viscous random-viscosity problem
+    (log10 μ a seeded Gaussian random field clipped to ±2, 1e-3 sphere with
+    ρg_z = -1): FP64 PT against FP32 PT sweeps inside FP64 defect
+    correction, both from the same initial state, with the per-field error of
+    the mixed-precision solution against the FP64 one."""
+    import numpy as np
+    from ptstokes import StokesModel
+    names = ["Pt", "Vx", "Vy", "Vz", "txx", "tyy", "tzz", "txy", "txz", "tyz"]
+    m = StokesModel("Stokes3D_grf", n=n, nout=nout).setup()
+    m.ctx.sync()
+    t0 = time.perf_counter()
+    r64 = m.step()
+    m.ctx.sync()
+    t64 = time.perf_counter() - t0
+    ref = {f: m.get(f) for f in names}
+    m.close()
+    m = StokesModel("Stokes3D_grf", n=n).setup()
+    m.ctx.sync()
+    t0 = time.perf_counter()
+    rdc = m.solve_dc(rtol_inner=rtol_inner, nout_inner=nout)
+    m.ctx.sync()
+    tdc = time.perf_counter() - t0
+    rel = {}
+    for f, v in ref.items():
+        a = m.get(f)
+        if f == "Pt":  # P is fixed up to a constant: compare interior cells, mean removed
+            a, v = a[1:-1, 1:-1, 1:-1], v[1:-1, 1:-1, 1:-1]
+            a, v = a - a.mean(), v - v.mean()
+        rel[f] = float(np.max(np.abs(a - v)) / np.max(np.abs(v)))
+    m.close()
+    return dict(workload=f"3D viscous Stokes, log10 mu GRF (seed 1234, corr. 0.5, sigma 1, clip 2) + 1e-3 sphere "
+                         f"with rhog_z=-1, nx=ny=nz={n}, stop ||res||_inf < 1e-8 (scaled max-norm)",
+                fp64_s=round(t64, 3), fp64_iters=r64["iters"], fp64_err=r64["err"],
+                dc_s=round(tdc, 3), dc_outer=rdc["outer"], dc_fp32_sweeps=rdc["inner_iters"], dc_err=rdc["err"],
+                dc_converged=rdc["converged"], time_ratio_dc_over_fp64=round(tdc / t64, 3),
+                dc_vs_fp64_max_rel_err=rel)
+
+
 def cpu_baseline(n, iters, threads):
     """The oracle (the reference's CPU kernels restated in C, OpenMP over the
     stencil loops) timed on the host cores at the same workload."""
@@ -118,6 +157,24 @@ def cpu_baseline(n, iters, threads):
     return dict(value=a_eff / t / 1e9, unit="GB/s", cores=threads, kind="port",
                 sample=f"{iters} PT iterations of 2D VE {n}x{n} FP64 (perf-script kernels, "
                        f"{threads} OpenMP threads) after setup; ms/it {t * 1e3:.1f}")
+
+
+def cpu_baseline_3d(n, iters, threads):
+    """Per-iteration throughput of the oracle on the 3D VE block."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    t0 = time.perf_counter()
+    o = Oracle("Stokes3D_ve_perf", n=n, threads=threads, nsmooth=1).setup()
+    t_setup = time.perf_counter() - t0
+    o.assign_tau()
+    o.iterate(1)
+    t0 = time.perf_counter()
+    o.iterate(iters)
+    t = (time.perf_counter() - t0) / iters
+    a_eff = 29.0 * n ** 3 * 8
+    return dict(value=a_eff / t / 1e9, unit="GB/s", cores=threads, kind="port", ms_per_it=t * 1e3,
+                sample=f"{iters} PT iterations of 3D VE {n}^3 FP64 (perf-script kernels, {threads} OpenMP threads) "
+                       f"after setup ({t_setup:.0f} s, 1 smoothing pass)")
 
 
 def load_traffic(key):
@@ -139,6 +196,8 @@ def main():
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--n-tts", type=int, default=127, help="time-to-tolerance grid (0: skip)")
+    ap.add_argument("--cpu-3d", action="store_true", help="also time the oracle on the 3D block")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -158,7 +217,18 @@ def main():
         extra["3d_ve"] = r3
         print(f"# 3D VE {args.n3d}^3: {r3['ms_per_it']:.4f} ms/it, T_eff {r3['T_eff_GBs']:.1f} GB/s",
               file=sys.stderr, flush=True)
+    if args.n_tts > 0:
+        tts = bench_tts(args.n_tts)
+        extra["tts_fp32_refined_vs_fp64"] = tts
+        print(f"# TTS {args.n_tts}^3: FP64 {tts['fp64_s']} s, FP32+DC {tts['dc_s']} s "
+              f"(ratio {tts['time_ratio_dc_over_fp64']})", file=sys.stderr, flush=True)
     cpu = None
+    if not args.no_cpu and not args.no_3d and args.cpu_3d:
+        threads = min(16, os.cpu_count() or 1)
+        c3 = cpu_baseline_3d(args.n3d, 1, threads)
+        extra["3d_ve"]["cpu_baseline"] = c3
+        extra["3d_ve"]["gpu_over_cpu"] = round(extra["3d_ve"]["T_eff_GBs"] / c3["value"], 1)
+        print(f"# CPU 3D: {c3['value']:.2f} GB/s ({c3['sample']})", file=sys.stderr, flush=True)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(args.n2d, args.cpu_iters, threads)

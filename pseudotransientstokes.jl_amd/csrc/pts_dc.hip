@@ -509,15 +509,36 @@ inline int gridN(int64_t tot) { return (int)std::min<int64_t>(65536, std::max<in
 
 }  // namespace
 
+// Resident blocks of `kern` with `threads` threads on the whole device.
+template <typename K>
+static int device_concurrency(K kern, int threads, int& cached) {
+    if (!cached) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0);
+        cached = std::max(1, cus) * std::max(1, per);
+    }
+    return cached;
+}
+
+// Rows (2D) or planes (3D) per block: about one round of resident blocks,
+// within [4, cap], so small grids still fill the device.
+static int march_length(int n, int columns, int conc, int cap) {
+    const int nch = std::max(1, conc / std::max(1, columns));
+    return std::max(4, std::min((n + nch - 1) / nch, cap));
+}
+
 hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim) {
+    static int conc2 = 0, conc3 = 0;
     if (ndim == 2) {
-        const int L = 128;
         const int ns = (in.nx + 1 + 61) / 62;
+        const int L = march_length(in.ny, ns, device_concurrency(k2_dc, 64, conc2), 128);
         const int nb = ns * ((in.ny + L - 1) / L);
         k2_dc<<<nb, 64, 0, s>>>(in, out, c, L, ns);
     } else {
-        const int Lz = 64;
         const int ntx = (in.nx + 1 + 61) / 62, nty = (in.ny + 1 + BY3 - 3) / (BY3 - 2);
+        const int Lz = march_length(in.nz, ntx * nty, device_concurrency(k3_dc, 64 * BY3, conc3), 64);
         const int nch = (in.nz + Lz - 1) / Lz;
         k3_dc<<<ntx * nty * nch, dim3(64, BY3), 0, s>>>(in, out, c, Lz, ntx, nty);
     }

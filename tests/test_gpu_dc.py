@@ -24,9 +24,10 @@ def test_dc_matches_fp64_solve(preset, kw):
     dc = StokesModel(preset, eps=1e-8, **kw).setup()
     res = dc.solve_dc()
     assert res["converged"] and res["err"] <= 1e-8, res
-    # each correction cuts the FP64 residual by orders of magnitude
+    # each correction cuts the FP64 residual by orders of magnitude, except
+    # the last, whose inner stop only asks for err below eps
     eo = res["err_outer"]
-    assert all(b < a * 1e-2 for a, b in zip(eo, eo[1:]))
+    assert all(b < max(a * 1e-2, 1e-8) for a, b in zip(eo, eo[1:])), eo
     nd = ref.p["ndim"]
     for f in FIELDS:
         if nd == 2 and f in ("Vz", "tzz", "txz", "tyz"):
