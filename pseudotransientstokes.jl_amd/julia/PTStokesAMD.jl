@@ -118,4 +118,46 @@ end
 pt_iterate!(c::Ctx, p::PtParams, n) =
     chk(c, ccall((:pts_pt_iterate, libpts), Cint, (Ptr{Cvoid}, Ref{PtParams}, Int64), c.h, Ref(p), n))
 
+# FP32 PT sweeps inside FP64 defect correction (viscous grids)
+struct DcParams
+    eps::Float64; norm_kind::Int32; max_outer::Int32; rtol_inner::Float64; nout_inner::Int64; iter_max_inner::Int64
+end
+mutable struct DcResult
+    outer::Int32; converged::Int32; inner_iters::Int64; err::Float64; err_outer::NTuple{32,Float64}
+end
+DcResult() = DcResult(0, 0, 0, 0.0, ntuple(_ -> 0.0, 32))
+function dc_solve!(c::Ctx, p::PtParams, ε; norm_kind=1, max_outer=20, rtol_inner=1e-4, nout_inner=200,
+                   iter_max_inner=10^7)
+    dp = Ref(DcParams(ε, norm_kind, max_outer, rtol_inner, nout_inner, iter_max_inner))
+    res = Ref(DcResult())
+    rc = ccall((:pts_dc_solve, libpts), Cint, (Ptr{Cvoid}, Ref{PtParams}, Ref{DcParams}, Ref{DcResult}),
+               c.h, Ref(p), dp, res)
+    rc in (0, -4) || chk(c, rc)
+    return res[]
+end
+
+# adjoint of the converged 2D viscous iteration and ∂J/∂ln μ
+function adjoint_set_source!(c::Ctx, dJ_dVx::Union{Nothing,Array{Float64}}, dJ_dVy::Union{Nothing,Array{Float64}})
+    px = dJ_dVx === nothing ? C_NULL : pointer(dJ_dVx)
+    py = dJ_dVy === nothing ? C_NULL : pointer(dJ_dVy)
+    GC.@preserve dJ_dVx dJ_dVy chk(c, ccall((:pts_adjoint_set_source, libpts), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Float64}, Int64), c.h, px,
+        dJ_dVx === nothing ? 0 : length(dJ_dVx), py, dJ_dVy === nothing ? 0 : length(dJ_dVy)))
+end
+function adjoint_solve!(c::Ctx, p::PtParams, ε, nout, iterMax)
+    sp = Ref(SolveParams(ε, nout, iterMax, 0, 0))
+    res = Ref(SolveResult())
+    rc = ccall((:pts_adjoint_solve, libpts), Cint, (Ptr{Cvoid}, Ref{PtParams}, Ref{SolveParams}, Ref{SolveResult}),
+               c.h, Ref(p), sp, res)
+    rc == -3 && error("NaN")
+    rc in (0, -4) || chk(c, rc)
+    return res[]
+end
+function grad_logμ(c::Ctx, p::PtParams, nx, ny)
+    g = zeros(Float64, nx, ny)
+    chk(c, ccall((:pts_grad_logmu, libpts), Cint, (Ptr{Cvoid}, Ref{PtParams}, Ptr{Float64}, Int64),
+                 c.h, Ref(p), g, length(g)))
+    return g
+end
+
 end # module

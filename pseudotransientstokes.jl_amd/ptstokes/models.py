@@ -62,6 +62,15 @@ R_OPT = 0.25
 RE_OPT = 9.0 * math.sqrt(3.0) * math.pi / 4.0
 
 
+def julia_linrange(a, b, n):
+    """The elements of Julia's LinRange(a, b, n): (1-t)·a + t·b with
+    t = (i-1)/(n-1), which differs from numpy.linspace in the last bits."""
+    if n == 1:
+        return np.array([a], dtype=np.float64)
+    t = np.arange(n, dtype=np.float64) / (n - 1)
+    return (1.0 - t) * a + t * b
+
+
 def config(preset, **overrides):
     d = dict(PRESETS[preset])
     d.update(overrides)
@@ -97,13 +106,14 @@ class StokesModel:
         self.timing = {}
 
     # coordinates of the cell centres (xc, yc[, zc]) as the scripts return them
+    # (xc = LinRange(dx/2, lx-dx/2, nx), Stokes2D_ve.jl:109)
     def coords(self):
         p = self.p
-        xc = np.linspace(self.dx / 2, p["lx"] - self.dx / 2, p["nx"])
-        yc = np.linspace(self.dy / 2, p["ly"] - self.dy / 2, p["ny"])
+        xc = julia_linrange(self.dx / 2, p["lx"] - self.dx / 2, p["nx"])
+        yc = julia_linrange(self.dy / 2, p["ly"] - self.dy / 2, p["ny"])
         if p["ndim"] == 2:
             return xc, yc
-        return xc, yc, np.linspace(self.dz / 2, p["lz"] - self.dz / 2, p["nz"])
+        return xc, yc, julia_linrange(self.dz / 2, p["lz"] - self.dz / 2, p["nz"])
 
     def setup(self):
         """ICs, 10x smooth! with swap, Musτ = maxloc(Mus) + BCs, iteration

@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 from oracle import Oracle
+from ptstokes.models import julia_linrange
 from _util import FIXTURES, PUBLISHED_ITERS, julia_isapprox, load_fixture, sample
 
 
@@ -18,9 +19,9 @@ def test_oracle_reproduces_golden_file(preset):
     assert julia_isapprox(S, P)
     # the restatement is exact: same bits
     assert np.array_equal(S, P), f"max |diff| {np.max(np.abs(S - P)):.3e}"
-    # X = cell centres (i-1/2)dx at the sampled indices
+    # X = the script's xc = LinRange(dx/2, lx-dx/2, nx) at the sampled indices, bit for bit
     dx = 10.0 / n
-    assert np.allclose(X, (np.array(idx) + 0.5) * dx, rtol=0, atol=1e-14)
+    assert np.array_equal(julia_linrange(dx / 2, 10.0 - dx / 2, n)[idx], X)
 
 
 def test_fixed_iteration_scripts_run_iterMax_plus_one():
