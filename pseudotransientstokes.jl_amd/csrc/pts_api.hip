@@ -39,10 +39,10 @@ struct pts_ctx {
     double* red_out = nullptr;
     double* h_red = nullptr;
     int64_t bytes = 0;
-    int rows_per_block = -128;  // 2D single-pass: rows per block (0: one round of resident waves; <0: that, capped)
-    int variant_2d = 1;         // 2D single-pass code variant (prefetch)
+    int rows_per_block = -32;   // 2D single-pass: rows per block (0: one round of resident blocks; <0: that, capped)
+    int variant_2d = 7;         // 2D single-pass code variant (prefetch, 8-wave strips)
     bool persist_2d = false;    // 2D single-pass: persistent waves looping over tiles
-    int planes_per_block = -64; // 3D single-pass: z planes per block (<0: one round of resident blocks, capped)
+    int planes_per_block = -24; // 3D single-pass: z planes per block (<0: one round of resident blocks, capped)
     int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
     // adjoint state λ (ping-pong by field id), the pass-1 scratch, the
@@ -52,9 +52,26 @@ struct pts_ctx {
     double* adj_src[2] = {};
     double* adj_grad = nullptr;
     int adj_cur = 0;
+    // Gdτ and dτ_Rho as produced by the last pts_iter_params from the current
+    // Musτ: the single-pass kernels may then derive them from Musτ (one array
+    // read instead of two).  Any write to Musτ, Gdτ or dτ_Rho clears dp_ok.
+    bool dp_ok = false;
+    bool dp_enabled = true;
+    StepScalars dp{};
+    // field allocations: the pointer handed to kernels may sit `stagger`
+    // bytes past the start of its hipMalloc block (see field_alloc)
+    std::map<void*, void*> alloc_base;
+    int64_t stagger = 4352;
+    int nallocs = 0;
     std::string err;
     // graphs of n single-pass iterations keyed by (n, parity of the start buffer)
-    std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
+    struct CachedGraph {
+        hipGraphExec_t exec;
+        Fields in, out;  // buffers and scalars the graph was captured with
+        StepScalars s;
+        bool dp;
+    };
+    std::map<std::pair<int64_t, int>, CachedGraph> graphs;
     int parity = 0;  // flips each single-pass iteration (which physical buffer is `cur`)
 };
 
@@ -72,6 +89,32 @@ int fail(pts_ctx* c, int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                                         \
             return fail((ctx), PTS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
     } while (0)
+
+// One field array of nalloc elements of `elem` bytes, zeroed.  With a
+// stagger of S bytes (a multiple of 256), the n-th array starts (n mod 16)·S
+// bytes into its block, so equal offsets of different arrays do not all fall
+// on the same HBM channel when the array size is a power of two.
+int field_alloc(pts_ctx* c, size_t elem, void** out) {
+    const size_t bytes = (size_t)c->nalloc * elem;
+    const size_t off = (size_t)((c->nallocs % 16) * c->stagger);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes + 16 * (size_t)c->stagger);
+    if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    void* q = (char*)p + off;
+    HIPCHK(c, hipMemsetAsync(q, 0, bytes, c->stream));
+    c->alloc_base[q] = p;
+    c->nallocs += 1;
+    c->bytes += (int64_t)(bytes + 16 * (size_t)c->stagger);
+    *out = q;
+    return PTS_OK;
+}
+
+void field_free(pts_ctx* c, void* q) {
+    if (!q) return;
+    auto it = c->alloc_base.find(q);
+    (void)hipFree(it != c->alloc_base.end() ? it->second : q);
+    if (it != c->alloc_base.end()) c->alloc_base.erase(it);
+}
 
 bool field_valid(const pts_ctx* c, int f) {
     if (f < 0 || f >= PTS_NFIELDS) return false;
@@ -91,11 +134,8 @@ int ensure(pts_ctx* c, int f) {
     if (!field_valid(c, f)) return fail(c, PTS_ERR_STATE, "field " + std::to_string(f) + " not valid for this grid");
     if (c->cur[f]) return PTS_OK;
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, (size_t)c->nalloc * sizeof(double));
-    if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    HIPCHK(c, hipMemsetAsync(p, 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    if (int rc = field_alloc(c, sizeof(double), &p)) return rc;
     c->cur[f] = (double*)p;
-    c->bytes += c->nalloc * (int64_t)sizeof(double);
     return PTS_OK;
 }
 
@@ -110,11 +150,8 @@ int ensure_list(pts_ctx* c, std::initializer_list<int> fs) {
 int ensure_alt(pts_ctx* c, int f) {
     if (c->alt[f]) return PTS_OK;
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, (size_t)c->nalloc * sizeof(double));
-    if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    HIPCHK(c, hipMemsetAsync(p, 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    if (int rc = field_alloc(c, sizeof(double), &p)) return rc;
     c->alt[f] = (double*)p;
-    c->bytes += c->nalloc * (int64_t)sizeof(double);
     c->alt_synced = false;
     return PTS_OK;
 }
@@ -135,6 +172,11 @@ Fields F(const pts_ctx* c) { return fields_of(c, c->cur); }
 // held by the ping-pong partner buffers.
 void touched(pts_ctx* c) { c->alt_synced = false; }
 
+// A write to field f (-1: any field) through the API.
+void wrote(pts_ctx* c, int f) {
+    if (f < 0 || f == PTS_MUSTAU || f == PTS_GDTAU || f == PTS_DTAU_RHO) c->dp_ok = false;
+}
+
 int setdev(pts_ctx* c) {
     HIPCHK(c, hipSetDevice(c->g.device));
     (void)hipGetLastError();  // launch checks below report only their own errors
@@ -142,7 +184,7 @@ int setdev(pts_ctx* c) {
 }
 
 StepScalars scal(double r, double G, double dt, double dx, double dy, double dz) {
-    StepScalars s;
+    StepScalars s{};
     s.r = r;
     s.Gdt = G * dt;
     s.dx = dx; s.dy = dy; s.dz = dz;
@@ -223,6 +265,8 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
+    if (const char* e = std::getenv("PTS_DERIVED_PARAMS")) c->dp_enabled = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_STAGGER_BYTES")) c->stagger = std::max<int64_t>(0, std::atoll(e)) / 256 * 256;
     hipError_t e = hipSetDevice(g->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void**)&c->red_partials, sizeof(double) * kRedBlocks * kRedVals);
@@ -241,19 +285,18 @@ int pts_destroy(pts_ctx* c) {
     if (!c) return PTS_OK;
     (void)hipSetDevice(c->g.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
     for (int q = 0; q < PTS_NFIELDS; ++q) {
-        if (c->cur[q]) (void)hipFree(c->cur[q]);
-        if (c->alt[q]) (void)hipFree(c->alt[q]);
-        if (c->cur32[q]) (void)hipFree(c->cur32[q]);
-        if (c->alt32[q] && c->alt32[q] != c->cur32[q]) (void)hipFree(c->alt32[q]);
-        if (c->adj[0][q]) (void)hipFree(c->adj[0][q]);
-        if (c->adj[1][q]) (void)hipFree(c->adj[1][q]);
-        if (c->adj_tmp[q]) (void)hipFree(c->adj_tmp[q]);
+        field_free(c, c->cur[q]);
+        field_free(c, c->alt[q]);
+        field_free(c, c->cur32[q]);
+        if (c->alt32[q] != c->cur32[q]) field_free(c, c->alt32[q]);
+        field_free(c, c->adj[0][q]);
+        field_free(c, c->adj[1][q]);
+        field_free(c, c->adj_tmp[q]);
     }
-    for (double* p : c->adj_src)
-        if (p) (void)hipFree(p);
-    if (c->adj_grad) (void)hipFree(c->adj_grad);
+    for (double* p : c->adj_src) field_free(c, p);
+    field_free(c, c->adj_grad);
     if (c->red_partials) (void)hipFree(c->red_partials);
     if (c->red_out) (void)hipFree(c->red_out);
     if (c->h_red) (void)hipHostFree(c->h_red);
@@ -309,7 +352,10 @@ static int copy_field(pts_ctx* c, int f, double* host, const double* chost, int6
     if (int rc = setdev(c)) return rc;
     if (int rc = ensure(c, f)) return rc;
     if (int rc = copy_dense(c, f, c->cur[f], host, chost, count, up)) return rc;
-    if (up) touched(c);
+    if (up) {
+        touched(c);
+        wrote(c, f);
+    }
     return PTS_OK;
 }
 
@@ -324,6 +370,7 @@ int pts_fill(pts_ctx* c, int f, double v) {
     field_shape(c->nd, c->nx, c->ny, c->nz, f, s);
     HIPCHK(c, launch_fill(c->stream, c->cur[f], v, F(c), s));
     touched(c);
+    wrote(c, f);
     return PTS_OK;
 }
 
@@ -337,6 +384,7 @@ int pts_copy(pts_ctx* c, int dst, int src) {
     if (a[0] != b[0] || a[1] != b[1] || a[2] != b[2]) return fail(c, PTS_ERR_ARG, "copy: shape mismatch");
     HIPCHK(c, launch_copy(c->stream, c->cur[dst], c->cur[src], F(c), a));
     touched(c);
+    wrote(c, dst);
     return PTS_OK;
 }
 
@@ -350,6 +398,8 @@ int pts_swap(pts_ctx* c, int a, int b) {
     std::swap(c->cur[a], c->cur[b]);
     std::swap(c->alt[a], c->alt[b]);
     touched(c);
+    wrote(c, a);
+    wrote(c, b);
     return PTS_OK;
 }
 
@@ -384,6 +434,7 @@ int pts_init_inclusion(pts_ctx* c, double lx, double ly, double lz, double mus0,
         if (c->cur[f]) HIPCHK(c, hipMemsetAsync(c->cur[f], 0, (size_t)c->nalloc * sizeof(double), c->stream));
     HIPCHK(c, launch_init_inclusion(c->stream, F(c), c->nd, lx, ly, lz, mus0, musi, rad2, ebg, rin, rout, igg));
     touched(c);
+    wrote(c, -1);
     return PTS_OK;
 }
 
@@ -411,6 +462,7 @@ int pts_smooth(pts_ctx* c, int dst, int src, double fact) {
     field_shape(c->nd, c->nx, c->ny, c->nz, src, s);
     HIPCHK(c, launch_smooth(c->stream, F(c), c->nd, c->cur[dst], c->cur[src], s, fact));
     touched(c);
+    wrote(c, dst);
     return PTS_OK;
 }
 
@@ -422,6 +474,7 @@ int pts_maxloc(pts_ctx* c, int dst, int src) {
     field_shape(c->nd, c->nx, c->ny, c->nz, src, s);
     HIPCHK(c, launch_maxloc(c->stream, F(c), c->nd, c->cur[dst], c->cur[src], s));
     touched(c);
+    wrote(c, dst);
     return PTS_OK;
 }
 
@@ -434,6 +487,7 @@ static int bc_dim(pts_ctx* c, int f, int dim) {
     field_shape(c->nd, c->nx, c->ny, c->nz, f, s);
     HIPCHK(c, launch_bc(c->stream, F(c), c->cur[f], s, dim));
     touched(c);
+    wrote(c, f);
     return PTS_OK;
 }
 int pts_bc_x(pts_ctx* c, int f) { return bc_dim(c, f, 0); }
@@ -445,6 +499,13 @@ int pts_iter_params(pts_ctx* c, double Vp, double G, double dt, double Re, doubl
     if (int rc = setdev(c)) return rc;
     if (int rc = ensure_list(c, {PTS_MUSTAU, PTS_GDTAU, PTS_DTAU_RHO})) return rc;
     HIPCHK(c, launch_iter_params(c->stream, F(c), c->nd, c->g.ve != 0, Vp, G, dt, Re, r, ml));
+    // the uniform sub-expressions of k_iter_params, evaluated the same way
+    c->dp.dp_ve = c->g.ve ? 1 : 0;
+    c->dp.ip_igdt = 1.0 / (G * dt);
+    c->dp.ip_k1 = Vp * ml / Re;
+    c->dp.ip_k2 = Vp * Vp;
+    c->dp.ip_r2 = r + 2.0;
+    c->dp_ok = true;
     return PTS_OK;
 }
 
@@ -652,16 +713,43 @@ static void swap_state(pts_ctx* c) {
 
 // Enqueue n iterations; for the 3D divide form, the last one stores ∇V of
 // the pre-update velocities when store_last_dv is set.
-static int enqueue_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_last_dv) {
+static bool use_dp(const pts_ctx* c) { return c->dp_ok && c->dp_enabled; }
+
+static StepScalars with_dp(const pts_ctx* c, StepScalars s) {
+    if (use_dp(c)) {
+        s.dp_ve = c->dp.dp_ve;
+        s.ip_igdt = c->dp.ip_igdt;
+        s.ip_k1 = c->dp.ip_k1;
+        s.ip_k2 = c->dp.ip_k2;
+        s.ip_r2 = c->dp.ip_r2;
+    }
+    return s;
+}
+
+static bool same_scal(const StepScalars& a, const StepScalars& b) {
+    return a.r == b.r && a.Gdt == b.Gdt && a.dx == b.dx && a.dy == b.dy && a.dz == b.dz && a._dx == b._dx &&
+           a._dy == b._dy && a._dz == b._dz && a.dp_ve == b.dp_ve && a.ip_igdt == b.ip_igdt && a.ip_k1 == b.ip_k1 &&
+           a.ip_k2 == b.ip_k2 && a.ip_r2 == b.ip_r2;
+}
+
+static bool same_fields(const Fields& a, const Fields& b) {
+    for (int q = 0; q < PTS_NFIELDS; ++q)
+        if (a.p[q] != b.p[q]) return false;
+    return a.sx == b.sx && a.sxy == b.sxy && a.nx == b.nx && a.ny == b.ny && a.nz == b.nz;
+}
+
+static int enqueue_iters(pts_ctx* c, const StepScalars& s0, int64_t n, bool store_last_dv) {
     const bool ve = c->g.ve != 0, mul = c->g.fd_mode == PTS_FD_MUL, bf = c->g.body_force != 0;
+    const bool dp = use_dp(c);
+    const StepScalars s = with_dp(c, s0);
     for (int64_t q = 0; q < n; ++q) {
         if (single_pass(c)) {
             if (c->nd == 2)
                 HIPCHK(c, launch2_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->rows_per_block,
-                                       c->variant_2d, c->persist_2d));
+                                       c->variant_2d, c->persist_2d, dp));
             else
                 HIPCHK(c, launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
-                                       c->tile_rows_3d));
+                                       c->tile_rows_3d, dp));
             swap_state(c);
         } else {
             if (store_last_dv && q == n - 1 && !mul) {
@@ -680,9 +768,18 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store
 static int run_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_last_dv, bool use_graph) {
     if (n <= 0) return PTS_OK;
     if (!use_graph || !single_pass(c) || n < 2) return enqueue_iters(c, s, n, store_last_dv);
-    // Capture n single-pass iterations once per (n, parity) and replay.
+    // Capture n single-pass iterations once per (n, parity) and replay; a
+    // graph captured with other buffers or scalars is captured again.
     auto key = std::make_pair(n, c->parity);
     auto it = c->graphs.find(key);
+    const Fields fin = F(c), fout = fields_of(c, c->alt);
+    const StepScalars sd = with_dp(c, s);
+    if (it != c->graphs.end() && !(same_fields(it->second.in, fin) && same_fields(it->second.out, fout) &&
+                                   same_scal(it->second.s, sd) && it->second.dp == use_dp(c))) {
+        (void)hipGraphExecDestroy(it->second.exec);
+        c->graphs.erase(it);
+        it = c->graphs.end();
+    }
     if (it == c->graphs.end()) {
         hipGraph_t graph;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -696,10 +793,10 @@ static int run_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_las
         if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         // capture already swapped the buffers n times: restore, then replay
         for (int64_t q = 0; q < n; ++q) swap_state(c);
-        c->graphs[key] = ex;
+        c->graphs[key] = pts_ctx::CachedGraph{ex, fin, fout, sd, use_dp(c)};
         it = c->graphs.find(key);
     }
-    HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+    HIPCHK(c, hipGraphLaunch(it->second.exec, c->stream));
     for (int64_t q = 0; q < n; ++q) swap_state(c);
     return PTS_OK;
 }
@@ -770,11 +867,8 @@ int pts_pt_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp,
 static int ensure32(pts_ctx* c, float** slot) {
     if (*slot) return PTS_OK;
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, (size_t)c->nalloc * sizeof(float));
-    if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    HIPCHK(c, hipMemsetAsync(p, 0, (size_t)c->nalloc * sizeof(float), c->stream));
+    if (int rc = field_alloc(c, sizeof(float), &p)) return rc;
     *slot = (float*)p;
-    c->bytes += c->nalloc * (int64_t)sizeof(float);
     return PTS_OK;
 }
 
@@ -899,11 +993,8 @@ static const int kAdj2[] = {PTS_PT, PTS_TXX, PTS_TYY, PTS_TXY, PTS_VX, PTS_VY};
 static int alloc_zero(pts_ctx* c, double** slot) {
     if (*slot) return PTS_OK;
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, (size_t)c->nalloc * sizeof(double));
-    if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    HIPCHK(c, hipMemsetAsync(p, 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    if (int rc = field_alloc(c, sizeof(double), &p)) return rc;
     *slot = (double*)p;
-    c->bytes += c->nalloc * (int64_t)sizeof(double);
     return PTS_OK;
 }
 

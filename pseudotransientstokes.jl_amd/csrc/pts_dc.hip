@@ -42,7 +42,7 @@ __device__ __forceinline__ float tau_visc_f(float t, float g, float m, float e) 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k2_dc(const Fields32 in, const Fields32 out, const StepScalars32 cs, int L,
                                             int nstrips) {
-    const int b = blockIdx.x;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int strip = b % nstrips, chunk = b / nstrips;
     const int lane = threadIdx.x;
     const int cx = strip * 62 - 1 + lane;
@@ -138,7 +138,7 @@ constexpr int BY3 = 12;
 __global__ __launch_bounds__(64 * BY3) void k3_dc(const Fields32 in, const Fields32 out, const StepScalars32 cs,
                                                   int Lz, int ntx, int nty) {
     __shared__ float sh[4][BY3][64];
-    const int b = blockIdx.x;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int ntile = ntx * nty;
     const int tile = b % ntile, chunk = b / ntile;
     const int tx = tile % ntx, ty = tile / ntx;

@@ -43,6 +43,15 @@ struct Ctx;
 // from_left(v)[lane] = v[lane-1]; from_right(v)[lane] = v[lane+1].  Lanes at
 // the wave ends receive 0.  DPP wave_shr/wave_shl (GFX9 family) move one
 // dword per instruction without touching LDS.
+// XCD-aware tile order.  The hardware deals consecutive blocks of a launch to
+// the 8 XCDs round-robin (block b runs on XCD b % 8); this bijection of
+// [0, n) gives each XCD one contiguous run of tiles, so neighbouring tiles
+// (which share halo rows and cache lines) meet in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int x = b & 7, i = b >> 3, q = n >> 3, rem = n & 7;
+    return x * q + min(x, rem) + i;
+}
+
 __device__ __forceinline__ double from_left(double v) {
     int lo = __double2loint(v), hi = __double2hiint(v);
     lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);
@@ -98,11 +107,26 @@ __device__ __forceinline__ void put_bc(T* V, int q, T v, int a, int na, int sa, 
     if (da && db) V[q + da * sa + db * sb] = v;
 }
 
-// Scalars of one PT iteration.
+// Scalars of one PT iteration.  dp kernels (`dp` launch flag) read Musτ and
+// derive Gdτ, dτ_Rho from it.
 struct StepScalars {
     double r, Gdt;
     double dx, dy, dz, _dx, _dy, _dz;
+    // constants of compute_iter_params! for kernels that derive Gdτ and
+    // dτ_Rho from Musτ instead of reading them (dp_ve: visco-elastic μ_eff)
+    int dp_ve;
+    double ip_igdt, ip_k1, ip_k2, ip_r2;
 };
+
+// dτ_Rho and Gdτ of one cell from Musτ, operation for operation as
+// compute_iter_params! (Stokes2D_ve.jl:28-33, Stokes2D.jl:27-31) evaluates
+// them, with the uniform sub-expressions 1/(G·dt), Vpdτ·max_l/Re, Vpdτ·Vpdτ
+// and r+2 precomputed on the host: bit-identical to the stored arrays.
+__device__ __forceinline__ void iter_params_of(double M, const StepScalars& c, double& rho, double& gd) {
+    const double mu_eff = c.dp_ve ? 1.0 / (1.0 / M + c.ip_igdt) : M;
+    rho = c.ip_k1 / mu_eff;
+    gd = c.ip_k2 / rho / c.ip_r2;
+}
 
 struct StepScalars32 {
     float r, _dx, _dy, _dz;
@@ -138,7 +162,7 @@ hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 // looping over tiles.  variant: 0 plain, 1 prefetch, 2 prefetch + non-temporal
 // stores, 3 non-temporal stores.
 hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
-                        bool mul, bool bf, int rows_per_block, int variant, bool persist);
+                        bool mul, bool bf, int rows_per_block, int variant, bool persist, bool dp);
 
 // 3D
 hipError_t launch3_compute_P(hipStream_t s, const Fields& f, const StepScalars& c);
@@ -151,7 +175,7 @@ hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
                                   bool fold_bc);
 // 3D single-pass iteration (ping-pong)
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
-                        bool mul, bool bf, int planes_per_block, int tile_rows);
+                        bool mul, bool bf, int planes_per_block, int tile_rows, bool dp);
 
 // Mixed-precision defect correction (pts_dc.hip)
 hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim);

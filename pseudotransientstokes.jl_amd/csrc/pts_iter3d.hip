@@ -21,14 +21,14 @@ namespace pts {
 
 namespace {
 
-template <bool VE, bool MUL, bool BF, int BY>
+template <bool VE, bool MUL, bool BF, int BY, bool DP>
 __global__ __launch_bounds__(64 * BY) void k3_iter(const Fields in, const Fields out, const StepScalars cs, int Lz,
                                                    int ntx, int nty) {
     __shared__ double sh[4][BY][64];  // Pt1, τyy1 (to row +1); τxy1, τyz1 (to row -1)
 
     int b = blockIdx.x;
     const int nb = gridDim.x;
-    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);  // contiguous tiles per XCD
+    b = xcd_tile(b, nb);  // contiguous tiles per XCD
     const int ntile = ntx * nty;
     const int tile = b % ntile, chunk = b / ntile;
     const int tx = tile % ntx, ty = tile / ntx;
@@ -40,6 +40,10 @@ __global__ __launch_bounds__(64 * BY) void k3_iter(const Fields in, const Fields
     const int ke = min(kb + Lz, nz);
     const bool own = lane >= 1 && lane <= 62 && w >= 1 && w <= BY - 2;
     const int wu = w > 0 ? w - 1 : 0, wd = w < BY - 1 ? w + 1 : BY - 1;
+    // The halo rows (one wave each) only feed their neighbour row: the top row
+    // its new Pt and τyy, the bottom row its new τxy and τyz.  They skip the
+    // loads those values do not need (wave-uniform branches).
+    const bool top = w == 0, bot = w == BY - 1, mid = !top && !bot;
 
     const bool cell = c >= 0 && c < nx && d >= 0 && d < ny;
     const bool cellU = c >= 0 && c < nx && d >= 1 && d <= ny;  // cell (c, d-1)
@@ -75,6 +79,7 @@ __global__ __launch_bounds__(64 * BY) void k3_iter(const Fields in, const Fields
     const double* __restrict__ Vx0 = in.p[PTS_VX];
     const double* __restrict__ Vy0 = in.p[PTS_VY];
     const double* __restrict__ Vz0 = in.p[PTS_VZ];
+    const double* __restrict__ Mtp = in.p[PTS_MUSTAU];
     const double* __restrict__ Gdp = in.p[PTS_GDTAU];
     const double* __restrict__ Mup = in.p[PTS_MUS];
     const double* __restrict__ Rhp = in.p[PTS_DTAU_RHO];
@@ -113,47 +118,59 @@ __global__ __launch_bounds__(64 * BY) void k3_iter(const Fields in, const Fields
         double txy0v = 0.0, txz0v = 0.0, tyz0v = 0.0, txyov = 0.0, txzov = 0.0, tyzov = 0.0;
         double Vx0c = 0.0, Vy0c = 0.0, GdU = 0.0, MuU = 0.0, RhU = 0.0, VxU = 0.0, VzUN = 0.0;
         if (cell) {
-            Pt0v = Pt0[o];
-            Gd = Gdp[o];
+            if constexpr (DP) {
+                iter_params_of(Mtp[o], cs, Rh, Gd);
+            } else {
+                Gd = Gdp[o];
+                if (mid) Rh = Rhp[o];
+            }
             Mu = Mup[o];
-            Rh = Rhp[o];
-            Vy0n = Vy0[o + sx];
             VzN = Vz0[o + sxy];
-            if constexpr (BF) rg = rgp[o];
+            if (!bot) {
+                Pt0v = Pt0[o];
+                Vy0n = Vy0[o + sx];
+            }
+            if (mid) {
+                if constexpr (BF) rg = rgp[o];
+            }
         }
-        if (cellU) {
-            GdU = Gdp[o - sx];
+        if (cellU && !top) {
+            if constexpr (DP) {
+                iter_params_of(Mtp[o - sx], cs, RhU, GdU);
+            } else {
+                GdU = Gdp[o - sx];
+                if (mid) RhU = Rhp[o - sx];
+            }
             MuU = Mup[o - sx];
-            RhU = Rhp[o - sx];
             VzUN = Vz0[o - sx + sxy];
         }
         if (vxv) Vx0c = Vx0[o];
-        if (vxU) VxU = Vx0[o - sx];
+        if (vxU && !top) VxU = Vx0[o - sx];
         if (vyv) Vy0c = Vy0[o];
         const bool hxx = txx_cd && kin2, hyy = tyy_cd && kin2, hzz = tzz_cd;
         const bool hxy = txy_cd && kin2, hxz = txz_cd && kin1, hyz = tyz_cd && kin1;
         const int oe = o - 1 - sx - sxy;  // edge-stress index (c-1, d-1, k-1)
-        if (hxx) {
+        if (hxx && mid) {
             tx0 = txx0[o - sx - sxy];
             if constexpr (VE) txo = txxo[o - sx - sxy];
         }
-        if (hyy) {
+        if (hyy && !bot) {
             ty0 = tyy0[o - 1 - sxy];
             if constexpr (VE) tyo = tyyo[o - 1 - sxy];
         }
-        if (hzz) {
+        if (hzz && mid) {
             tz0 = tzz0[o - 1 - sx];
             if constexpr (VE) tzo = tzzo[o - 1 - sx];
         }
-        if (hxy) {
+        if (hxy && !top) {
             txy0v = txy0[oe];
             if constexpr (VE) txyov = txyo[oe];
         }
-        if (hxz) {
+        if (hxz && mid) {
             txz0v = txz0[oe];
             if constexpr (VE) txzov = txzo[oe];
         }
-        if (hyz) {
+        if (hyz && !top) {
             tyz0v = tyz0[oe];
             if constexpr (VE) tyzov = tyzo[oe];
         }
@@ -287,14 +304,14 @@ __global__ __launch_bounds__(64 * BY) void k3_iter(const Fields in, const Fields
 }
 
 // Resident blocks of one k3_iter instance on the whole device.
-template <bool VE, bool MUL, bool BF, int BY>
+template <bool VE, bool MUL, bool BF, int BY, bool DP>
 int concurrency3() {
     static int cached = 0;
     if (!cached) {
         int dev = 0, cus = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k3_iter<VE, MUL, BF, BY>, 64 * BY, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k3_iter<VE, MUL, BF, BY, DP>, 64 * BY, 0);
         cached = std::max(1, cus) * std::max(1, per);
     }
     return cached;
@@ -302,7 +319,7 @@ int concurrency3() {
 
 // Lz planes per block; Lz < 0: about one round of resident blocks, within
 // [4, -Lz] planes (small grids get short columns, large ones long marches).
-template <bool VE, bool MUL, bool BF>
+template <bool VE, bool MUL, bool BF, bool DP>
 void go3(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int Lz, int by) {
     const int ntx = (in.nx + 1 + 61) / 62;
     auto launch = [&](auto byc) {
@@ -310,29 +327,44 @@ void go3(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& 
         const int nty = (in.ny + 1 + (BY - 2) - 1) / (BY - 2);
         int L = Lz;
         if (L < 0) {
-            const int nch = std::max(1, concurrency3<VE, MUL, BF, BY>() / (ntx * nty));
+            const int nch = std::max(1, concurrency3<VE, MUL, BF, BY, DP>() / (ntx * nty));
             L = std::max(4, std::min((in.nz + nch - 1) / nch, -Lz));
         }
         L = std::max(1, L);
         const int nch = (in.nz + L - 1) / L;
-        k3_iter<VE, MUL, BF, BY><<<ntx * nty * nch, dim3(64, BY), 0, s>>>(in, out, c, L, ntx, nty);
+        k3_iter<VE, MUL, BF, BY, DP><<<ntx * nty * nch, dim3(64, BY), 0, s>>>(in, out, c, L, ntx, nty);
     };
-    if (by == 8) launch(std::integral_constant<int, 8>{});
-    else if (by == 16) launch(std::integral_constant<int, 16>{});
-    else launch(std::integral_constant<int, 12>{});
+    if constexpr (DP) {
+        launch(std::integral_constant<int, 12>{});
+    } else {
+        if (by == 8) launch(std::integral_constant<int, 8>{});
+        else if (by == 16) launch(std::integral_constant<int, 16>{});
+        else launch(std::integral_constant<int, 12>{});
+    }
 }
 
 }  // namespace
 
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve, bool mul,
-                        bool bf, int Lz, int by) {
+                        bool bf, int Lz, int by, bool dp) {
     if (Lz == 0) Lz = 1;
+    // dp: derive Gdτ, dτ_Rho from Musτ (tile rows 12 only; the others read them)
+    if (dp && (by != 8 && by != 16)) {
+        if (ve) {
+            if (mul) { if (bf) go3<true, true, true, true>(s, in, out, c, Lz, 12); else go3<true, true, false, true>(s, in, out, c, Lz, 12); }
+            else { if (bf) go3<true, false, true, true>(s, in, out, c, Lz, 12); else go3<true, false, false, true>(s, in, out, c, Lz, 12); }
+        } else {
+            if (mul) { if (bf) go3<false, true, true, true>(s, in, out, c, Lz, 12); else go3<false, true, false, true>(s, in, out, c, Lz, 12); }
+            else { if (bf) go3<false, false, true, true>(s, in, out, c, Lz, 12); else go3<false, false, false, true>(s, in, out, c, Lz, 12); }
+        }
+        return hipGetLastError();
+    }
     if (ve) {
-        if (mul) { if (bf) go3<true, true, true>(s, in, out, c, Lz, by); else go3<true, true, false>(s, in, out, c, Lz, by); }
-        else { if (bf) go3<true, false, true>(s, in, out, c, Lz, by); else go3<true, false, false>(s, in, out, c, Lz, by); }
+        if (mul) { if (bf) go3<true, true, true, false>(s, in, out, c, Lz, by); else go3<true, true, false, false>(s, in, out, c, Lz, by); }
+        else { if (bf) go3<true, false, true, false>(s, in, out, c, Lz, by); else go3<true, false, false, false>(s, in, out, c, Lz, by); }
     } else {
-        if (mul) { if (bf) go3<false, true, true>(s, in, out, c, Lz, by); else go3<false, true, false>(s, in, out, c, Lz, by); }
-        else { if (bf) go3<false, false, true>(s, in, out, c, Lz, by); else go3<false, false, false>(s, in, out, c, Lz, by); }
+        if (mul) { if (bf) go3<false, true, true, false>(s, in, out, c, Lz, by); else go3<false, true, false, false>(s, in, out, c, Lz, by); }
+        else { if (bf) go3<false, false, true, false>(s, in, out, c, Lz, by); else go3<false, false, false, false>(s, in, out, c, Lz, by); }
     }
     return hipGetLastError();
 }

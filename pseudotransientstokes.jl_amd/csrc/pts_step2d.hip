@@ -188,8 +188,9 @@ struct Row2 {
     double Pt, txx, tyy, txo, tyo, Gd, Mu, Rh, VyN, rg, Vx, txy, txyo;
 };
 
-template <bool VE, bool BF>
-__device__ __forceinline__ void load_row2(Row2& w, const Fields& in, int o, int sx, bool cin, bool fin, bool vtx) {
+template <bool VE, bool BF, bool DP>
+__device__ __forceinline__ void load_row2(Row2& w, const Fields& in, const StepScalars& c, int o, int sx, bool cin,
+                                          bool fin, bool vtx) {
     w = Row2{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (cin) {
         w.Pt = in.p[PTS_PT][o];
@@ -199,9 +200,13 @@ __device__ __forceinline__ void load_row2(Row2& w, const Fields& in, int o, int 
             w.txo = in.p[PTS_TXX_O][o];
             w.tyo = in.p[PTS_TYY_O][o];
         }
-        w.Gd = in.p[PTS_GDTAU][o];
+        if constexpr (DP) {
+            iter_params_of(in.p[PTS_MUSTAU][o], c, w.Rh, w.Gd);
+        } else {
+            w.Gd = in.p[PTS_GDTAU][o];
+            w.Rh = in.p[PTS_DTAU_RHO][o];
+        }
         w.Mu = in.p[PTS_MUS][o];
-        w.Rh = in.p[PTS_DTAU_RHO][o];
         w.VyN = in.p[PTS_VY][o + sx];
         if constexpr (BF) w.rg = in.p[PTS_RHOG][o];
     }
@@ -217,8 +222,9 @@ __device__ __forceinline__ void load_row2(Row2& w, const Fields& in, int o, int 
 // exactly the row it needs.  Lanes outside the grid then hold copies of edge
 // values; no stored output depends on them (each output's operands come from
 // lanes and rows inside its own validity range).
-template <bool VE, bool BF>
-__device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, int cx, int J, int nx, int sx) {
+template <bool VE, bool BF, bool DP>
+__device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, const StepScalars& c, int cx, int J,
+                                                  int nx, int sx) {
     const int oc = min(max(cx, 0), nx - 1) + sx * J;
     const int of = min(max(cx, 0), nx) + sx * J;
     const int ov = min(max(cx - 1, 0), nx - 2) + sx * max(J - 1, 0);
@@ -231,9 +237,13 @@ __device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, int
         w.tyo = in.p[PTS_TYY_O][oc];
         w.txyo = in.p[PTS_TXY_O][ov];
     }
-    w.Gd = in.p[PTS_GDTAU][oc];
+    if constexpr (DP) {
+        iter_params_of(in.p[PTS_MUSTAU][oc], c, w.Rh, w.Gd);
+    } else {
+        w.Gd = in.p[PTS_GDTAU][oc];
+        w.Rh = in.p[PTS_DTAU_RHO][oc];
+    }
     w.Mu = in.p[PTS_MUS][oc];
-    w.Rh = in.p[PTS_DTAU_RHO][oc];
     w.VyN = in.p[PTS_VY][oc + sx];
     if constexpr (BF) w.rg = in.p[PTS_RHOG][oc];
     w.Vx = in.p[PTS_VX][of];
@@ -261,10 +271,18 @@ __device__ __forceinline__ void st(double* p, double v) {
 // of row J+1 are issued before row J is computed; with NT the stores bypass
 // the caches (the outputs are not read again in this launch).
 // ---------------------------------------------------------------------------
-template <bool VE, bool MUL, bool BF, bool PF, bool NT>
-__global__ __launch_bounds__(64) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
-                                              int nstrips, int ntiles) {
-    const int lane = threadIdx.x;
+//
+// NW > 1: a block of NW waves owns a strip of 64·NW − 2 columns (only its
+// first and last lane are halo), so a row of each array is read as one
+// contiguous run of 64·NW values; values crossing the wave boundaries inside
+// the block go through LDS (two barriers per row step).
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP>
+__global__ __launch_bounds__(64 * NW) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
+                                                   int nstrips, int ntiles) {
+    constexpr int W = 64 * NW - 2;  // output columns per strip
+    constexpr int NXL = 6;          // values passed to the right neighbour wave
+    __shared__ double sL[NW > 1 ? NW : 1][NXL], sR[NW > 1 ? NW : 1], sT[NW > 1 ? NW : 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, gl = threadIdx.x;
     const int nx = in.nx, ny = in.ny, sx = in.sx;
     double* __restrict__ Pt1 = out.p[PTS_PT];
     double* __restrict__ txx1 = out.p[PTS_TXX];
@@ -280,12 +298,12 @@ __global__ __launch_bounds__(64) void k2_iter(const Fields in, const Fields out,
         // XCD-aware order: the hardware deals consecutive blocks to the 8 XCDs
         // round-robin; remap so each XCD gets a contiguous run of tiles and the
         // halo columns shared by neighbouring strips stay in one L2.
-        if (nb >= ntiles && (ntiles & 7) == 0) b = (b & 7) * (ntiles >> 3) + (b >> 3);
+        if (nb >= ntiles) b = xcd_tile(b, ntiles);
         const int strip = b % nstrips, chunk = b / nstrips;
-        const int cx = strip * 62 - 1 + lane;
+        const int cx = strip * W - 1 + gl;
         const int jb = chunk * L;
         const int je = min(jb + L, ny);
-        const bool own = lane >= 1 && lane <= 62;
+        const bool own = gl >= 1 && gl <= W;
         const bool cin = cx >= 0 && cx < nx;       // cell column
         const bool fin = cx >= 0 && cx <= nx;      // Vx face
         const bool vin = cx >= 1 && cx <= nx - 1;  // vertex column cx-1 / interior Vx face
@@ -301,13 +319,34 @@ __global__ __launch_bounds__(64) void k2_iter(const Fields in, const Fields out,
             const int o = cx + sx * J;
             const bool vtx = vin && J >= 1;
             // lane exchanges (whole wave active)
-            const double VxR = from_right(cur.Vx);
-            const double GdL = from_left(cur.Gd);
-            const double MuL = from_left(cur.Mu);
-            const double VyCL = from_left(VyC);
-            const double txx1PL = from_left(txx1P);
-            const double Pt1PL = from_left(Pt1P);
-            const double RhPL = from_left(RhP);
+            double VxR = from_right(cur.Vx);
+            double GdL = from_left(cur.Gd);
+            double MuL = from_left(cur.Mu);
+            double VyCL = from_left(VyC);
+            double txx1PL = from_left(txx1P);
+            double Pt1PL = from_left(Pt1P);
+            double RhPL = from_left(RhP);
+            if constexpr (NW > 1) {  // across the wave boundaries of the block
+                if (lane == 63) {
+                    sL[wv][0] = cur.Gd;
+                    sL[wv][1] = cur.Mu;
+                    sL[wv][2] = VyC;
+                    sL[wv][3] = txx1P;
+                    sL[wv][4] = Pt1P;
+                    sL[wv][5] = RhP;
+                }
+                if (lane == 0) sR[wv] = cur.Vx;
+                __syncthreads();
+                if (lane == 0 && wv > 0) {
+                    GdL = sL[wv - 1][0];
+                    MuL = sL[wv - 1][1];
+                    VyCL = sL[wv - 1][2];
+                    txx1PL = sL[wv - 1][3];
+                    Pt1PL = sL[wv - 1][4];
+                    RhPL = sL[wv - 1][5];
+                }
+                if (lane == 63 && wv < NW - 1) VxR = sR[wv + 1];
+            }
 
             // cell (c, J)
             const double dVx = VxR - cur.Vx, dVy = cur.VyN - VyC;
@@ -326,7 +365,12 @@ __global__ __launch_bounds__(64) void k2_iter(const Fields in, const Fields out,
             const double am = (MuPl + MuP + MuL + cur.Mu) * 0.25;
             const double e = 0.5 * (dfd<MUL>(cur.Vx - VxP, dy, _dy) + dfd<MUL>(VyC - VyCL, dx, _dx));
             const double txy1n = tau_upd<VE>(cur.txy, cur.txyo, ag, am, Gdt, e);
-            const double txy1nR = from_right(txy1n);
+            double txy1nR = from_right(txy1n);
+            if constexpr (NW > 1) {
+                if (lane == 0) sT[wv] = txy1n;
+                __syncthreads();
+                if (lane == 63 && wv < NW - 1) txy1nR = sT[wv + 1];
+            }
 
             // stores of the new cell/vertex values owned by this tile
             if (own && cin && J >= jb && J < je) {
@@ -390,17 +434,17 @@ __global__ __launch_bounds__(64) void k2_iter(const Fields in, const Fields out,
         };
         Row2 ra, rb;
         if constexpr (PF) {
-            load_row2_clamped<VE, BF>(ra, in, cx, j0, nx, sx);
+            load_row2_clamped<VE, BF, DP>(ra, in, c, cx, j0, nx, sx);
             for (int J = j0; J <= j1; J += 2) {
-                if (J + 1 <= j1) load_row2_clamped<VE, BF>(rb, in, cx, J + 1, nx, sx);
+                if (J + 1 <= j1) load_row2_clamped<VE, BF, DP>(rb, in, c, cx, J + 1, nx, sx);
                 step(J, ra);
                 if (J + 1 > j1) break;
-                if (J + 2 <= j1) load_row2_clamped<VE, BF>(ra, in, cx, J + 2, nx, sx);
+                if (J + 2 <= j1) load_row2_clamped<VE, BF, DP>(ra, in, c, cx, J + 2, nx, sx);
                 step(J + 1, rb);
             }
         } else {
             for (int J = j0; J <= j1; ++J) {
-                load_row2<VE, BF>(ra, in, cx + sx * J, sx, cin, fin, vin && J >= 1);
+                load_row2<VE, BF, DP>(ra, in, c, cx + sx * J, sx, cin, fin, vin && J >= 1);
                 step(J, ra);
             }
         }
@@ -441,24 +485,25 @@ hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
     return hipGetLastError();
 }
 
-// Resident single-wave blocks of one k2_iter instance on the whole device.
-template <bool VE, bool MUL, bool BF, bool PF, bool NT>
+// Resident blocks of one k2_iter instance on the whole device.
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP>
 static int concurrency2() {
     static int cached = 0;
     if (!cached) {
         int dev = 0, cus = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k2_iter<VE, MUL, BF, PF, NT>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k2_iter<VE, MUL, BF, PF, NT, NW, DP>, 64 * NW, 0);
         cached = std::max(1, cus) * std::max(1, per);
     }
     return cached;
 }
 
-template <bool VE, bool MUL, bool BF, bool PF, bool NT>
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW = 1, bool DP = false>
 static void go2v(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, bool persist) {
-    const int ns = (in.nx + 1 + 61) / 62;
-    const int conc = concurrency2<VE, MUL, BF, PF, NT>();
+    constexpr int W = 64 * NW - 2;
+    const int ns = (in.nx + 1 + W - 1) / W;
+    const int conc = concurrency2<VE, MUL, BF, PF, NT, NW, DP>();
     if (L < 1) {  // about one round of resident waves (L < 0: within [4, -L] rows)
         const int nch = std::max(1, conc / ns);
         const int cap = -L;
@@ -467,28 +512,37 @@ static void go2v(hipStream_t s, const Fields& in, const Fields& out, const StepS
     }
     const int ntiles = ns * ((in.ny + L - 1) / L);
     const int grid = persist ? std::min(ntiles, conc) : ntiles;
-    k2_iter<VE, MUL, BF, PF, NT><<<grid, 64, 0, s>>>(in, out, c, L, ns, ntiles);
+    k2_iter<VE, MUL, BF, PF, NT, NW, DP><<<grid, 64 * NW, 0, s>>>(in, out, c, L, ns, ntiles);
 }
 
+// dp: derive Gdτ, dτ_Rho from Musτ (variants 1 and 7; the others read them)
 template <bool VE, bool MUL, bool BF>
 static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, int variant,
-                bool persist) {
+                bool persist, bool dp) {
+    if (dp && variant == 7) return go2v<VE, MUL, BF, true, false, 8, true>(s, in, out, c, L, persist);
+    if (dp && variant == 1) return go2v<VE, MUL, BF, true, false, 1, true>(s, in, out, c, L, persist);
     switch (variant) {
         case 0: go2v<VE, MUL, BF, false, false>(s, in, out, c, L, persist); break;
         case 2: go2v<VE, MUL, BF, true, true>(s, in, out, c, L, persist); break;
         case 3: go2v<VE, MUL, BF, false, true>(s, in, out, c, L, persist); break;
+        case 4: go2v<VE, MUL, BF, true, false, 4>(s, in, out, c, L, persist); break;
+        case 5: go2v<VE, MUL, BF, true, false, 2>(s, in, out, c, L, persist); break;
+        case 6: go2v<VE, MUL, BF, true, true, 4>(s, in, out, c, L, persist); break;
+        case 7: go2v<VE, MUL, BF, true, false, 8>(s, in, out, c, L, persist); break;
+        case 8: go2v<VE, MUL, BF, true, true, 8>(s, in, out, c, L, persist); break;
+        case 9: go2v<VE, MUL, BF, true, false, 16>(s, in, out, c, L, persist); break;
         default: go2v<VE, MUL, BF, true, false>(s, in, out, c, L, persist); break;
     }
 }
 
 hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
-                        bool mul, bool bf, int L, int variant, bool persist) {
+                        bool mul, bool bf, int L, int variant, bool persist, bool dp) {
     if (ve) {
-        if (mul) { if (bf) go2<true, true, true>(s, in, out, c, L, variant, persist); else go2<true, true, false>(s, in, out, c, L, variant, persist); }
-        else { if (bf) go2<true, false, true>(s, in, out, c, L, variant, persist); else go2<true, false, false>(s, in, out, c, L, variant, persist); }
+        if (mul) { if (bf) go2<true, true, true>(s, in, out, c, L, variant, persist, dp); else go2<true, true, false>(s, in, out, c, L, variant, persist, dp); }
+        else { if (bf) go2<true, false, true>(s, in, out, c, L, variant, persist, dp); else go2<true, false, false>(s, in, out, c, L, variant, persist, dp); }
     } else {
-        if (mul) { if (bf) go2<false, true, true>(s, in, out, c, L, variant, persist); else go2<false, true, false>(s, in, out, c, L, variant, persist); }
-        else { if (bf) go2<false, false, true>(s, in, out, c, L, variant, persist); else go2<false, false, false>(s, in, out, c, L, variant, persist); }
+        if (mul) { if (bf) go2<false, true, true>(s, in, out, c, L, variant, persist, dp); else go2<false, true, false>(s, in, out, c, L, variant, persist, dp); }
+        else { if (bf) go2<false, false, true>(s, in, out, c, L, variant, persist, dp); else go2<false, false, false>(s, in, out, c, L, variant, persist, dp); }
     }
     return hipGetLastError();
 }

@@ -76,6 +76,57 @@ def test_upload_after_iterate_resyncs_dirichlet_values():
     m.close()
 
 
+@pytest.mark.parametrize("preset,kw", [("Stokes2D_ve_perf", dict(nx=130, ny=61)),
+                                       ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19)),
+                                       ("Stokes2D", dict(n=70))])
+def test_uploaded_iteration_parameters_are_read(preset, kw):
+    # After setup the kernels derive Gdτ and dτ_Rho from Musτ; once the user
+    # writes them (or Musτ) directly, the stored arrays must be used.
+    m, o = pair(preset, **kw)
+    randomise(m, o, seed=3)
+    m.iterate(2)
+    o.iterate(2)
+    assert compare(m, o, STATE) == []
+    rng = np.random.default_rng(4)
+    for f, lo, hi in (("Gdtau", 0.05, 0.5), ("dtau_Rho", 0.001, 0.02)):
+        a = rng.uniform(lo, hi, size=o.shape(f))
+        o.set(f, a)
+        m.ctx.upload(f, a)
+    m.iterate(3)
+    o.iterate(3)
+    assert compare(m, o, STATE) == []
+    mt = o.get("Mustau") * rng.uniform(0.5, 2.0, size=o.shape("Mustau"))
+    o.set("Mustau", mt)
+    m.ctx.upload("Mustau", mt)  # Musτ no longer matches Gdτ / dτ_Rho
+    m.iterate(3)
+    o.iterate(3)
+    assert compare(m, o, STATE) == []
+    m.close()
+
+
+def test_cached_graphs_follow_new_scalars():
+    # pt_solve replays captured graphs; a call with other scalars (or after the
+    # iteration parameters changed) must not replay the old ones
+    from ptstokes import Context
+    a, _ = pair("Stokes2D_ve_perf", n=50)
+    b, _ = pair("Stokes2D_ve_perf", n=50)
+    p1 = a.pt_params()
+    p2 = Context.pt_params(0.7, 1.0, 0.5, a.dx, a.dy)
+    for params in (p1, p2, p1):
+        a.ctx.pt_solve(params, 1e-300, 10, 29, 0, use_graph=True)  # 30 iterations, no convergence
+        b.ctx.pt_iterate(params, 30)
+    a.ctx.iter_params(a.Vpdtau * 0.9, 1.0, 1.0, a.p["Re"], a.p["r"], a.max_l)
+    b.ctx.iter_params(b.Vpdtau * 0.9, 1.0, 1.0, b.p["Re"], b.p["r"], b.max_l)
+    a.ctx.pt_solve(p1, 1e-300, 10, 29, 0, use_graph=True)
+    b.ctx.pt_iterate(p1, 30)
+    for f in STATE:
+        if f in ("Vz", "tzz", "txz", "tyz"):
+            continue
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
+
+
 VARIANTS = [
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_3D_TWO_KERNEL": "1"}),
     ("Stokes3D_ve_multixpu", dict(n=21), {"PTS_3D_TWO_KERNEL": "1"}),
@@ -89,10 +140,27 @@ VARIANTS = [
     ("Stokes2D_ve_perf", dict(nx=130, ny=70), {"PTS_ROWS_PER_BLOCK": "0"}),
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=80), {"PTS_PLANES_PER_BLOCK": "64"}),
     ("Stokes3D_ve_perf", dict(nx=40, ny=30, nz=50), {"PTS_PLANES_PER_BLOCK": "-16"}),
+    ("Stokes2D_ve_perf", dict(nx=130, ny=70), {"PTS_2D_VARIANT": "0"}),
+    ("Stokes2D", dict(nx=70, ny=41), {"PTS_2D_VARIANT": "2"}),
+    ("Stokes2D_ve", dict(nx=70, ny=41), {"PTS_2D_VARIANT": "3"}),
+    ("Stokes2D_ve_perf", dict(nx=600, ny=77), {"PTS_2D_VARIANT": "4"}),
+    ("Stokes2D_ve_perf", dict(nx=20, ny=33), {"PTS_2D_VARIANT": "4"}),
+    ("Stokes2D", dict(nx=253, ny=40, rhog_in=-1.0), {"PTS_2D_VARIANT": "4"}),
+    ("Stokes2D_ve", dict(nx=509, ny=31), {"PTS_2D_VARIANT": "4", "PTS_ROWS_PER_BLOCK": "7"}),
+    ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_2D_VARIANT": "5"}),
+    ("Stokes2D_ve_perf", dict(nx=300, ny=290), {"PTS_2D_PERSIST": "1"}),
+    ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_2D_VARIANT": "1"}),
+    ("Stokes2D", dict(nx=1100, ny=45, rhog_in=-1.0), {"PTS_2D_VARIANT": "7"}),
+    ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_DERIVED_PARAMS": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_DERIVED_PARAMS": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_STAGGER_BYTES": "0"}),
+    ("Stokes3D_ve_multixpu", dict(n=17), {"PTS_STAGGER_BYTES": "65792"}),
 ]
 
 
-@pytest.mark.parametrize("preset,kw,env", VARIANTS, ids=[f"{p}-{list(e)[0]}={list(e.values())[0]}" for p, _, e in VARIANTS])
+@pytest.mark.parametrize("preset,kw,env", VARIANTS,
+                         ids=[f"{p}-" + "-".join(f"{k}={v}" for k, v in e.items()) + "-" + "-".join(f"{k}{v}" for k, v in kw.items())
+                              for p, kw, e in VARIANTS])
 def test_launch_variants_bit_exact(preset, kw, env, monkeypatch):
     for k, v in env.items():
         monkeypatch.setenv(k, v)
