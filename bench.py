@@ -142,6 +142,70 @@ def bench_tts(n, nout=500, rtol_inner=1e-4):
                 dc_vs_fp64_max_rel_err=rel)
 
 
+def bench_c1(n=127, threads=16):
+    """2D viscous Stokes (Stokes2D.jl, divide form) at 127²: 1e-3 inclusion of
+    radius 1 with ρg_y = -1 inside, free-slip walls (εbg = 0), Re and r at the
+    1D dispersion optimum, converged to ‖res‖∞ < 1e-8 with a check every 200
+    iterations: time to solution on the GPU and with the oracle on the host
+    cores (same iterations, same result bits).  CFL is the script's 0.9/√2:
+    at 0.95/√2 this problem diverges (NaN after ~2200 iterations)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    from ptstokes import StokesModel
+    from ptstokes.models import R_OPT, RE_OPT
+    kw = dict(n=n, CFL=0.9 / math.sqrt(2), Re=RE_OPT, r=R_OPT, rhog_in=-1.0, ebg=0.0, norm_kind=1, nout=200,
+              eps=1e-8)
+    m = StokesModel("Stokes2D", **kw).setup()
+    m.ctx.sync()
+    t0 = time.perf_counter()
+    r = m.step()
+    m.ctx.sync()
+    t_gpu = time.perf_counter() - t0
+    pt_gpu = m.get("Pt")
+    m.close()
+    okw = dict(n=n, perf=0, ve=0, CFL=kw["CFL"], Re=RE_OPT, r=R_OPT, rhog_in=-1.0, ebg=0.0, norm_inf=1, nout=200,
+               eps=1e-8, threads=threads)
+    o = Oracle("Stokes2D", **okw).setup()
+    t0 = time.perf_counter()
+    ro = o.solve()
+    t_cpu = time.perf_counter() - t0
+    import numpy as np
+    same = bool(np.array_equal(o.get("Pt"), pt_gpu))
+    return dict(workload="2D viscous Stokes 127^2, divide form, inclusion with rho_g=-1, free slip, "
+                         "||res||_inf < 1e-8 checked every 200 iterations",
+                gpu_s=round(t_gpu, 4), gpu_iters=r["iters"], cpu_s=round(t_cpu, 4), cpu_iters=ro["ittot"],
+                cpu_threads=threads, speedup=round(t_cpu / t_gpu, 1), identical_result=same)
+
+
+def bench_c5(n=2047, steps=200):
+    """Per-iteration cost of the forward and adjoint PT sweeps of the 2D
+    viscous inversion problem (log μ a seeded GRF) at n²."""
+    import numpy as np
+    from ptstokes.adjoint import Inversion
+    ev = HipEvents()
+    inv = Inversion(n=n, npts=256, eps=1e-8)  # set up with the seed-1234 field
+    m, c = inv.model, inv.ctx
+    fwd_ms, _ = time_iterations(m, steps, 10, ev)
+    g = np.zeros((n + 1, n))
+    g[inv.points, n - 1] = 1.0
+    c.adjoint_set_source(g, None)
+    c.adjoint_iterate(inv.params, 10)
+    s = c.stream()
+    a, b = ev.event(), ev.event()
+    ev.record(a, s)
+    c.adjoint_iterate(inv.params, steps)
+    ev.record(b, s)
+    adj_ms = ev.elapsed_ms(a, b)
+    inv.close()
+    a_eff = 15.0 * n * n * 8
+    f_it, a_it = fwd_ms / steps, adj_ms / steps
+    return dict(workload=f"2D viscous Stokes {n}^2, log mu GRF (seed 1234), forward and adjoint PT iterations",
+                forward_ms_per_it=round(f_it, 4), forward_T_eff_GBs=round(a_eff / f_it / 1e6, 1),
+                adjoint_ms_per_it=round(a_it, 4), adjoint_T_eff_GBs=round(a_eff / a_it / 1e6, 1),
+                A_eff_bytes=a_eff, note="T_eff with A_eff = 15 arrays (6 state read+written, 3 read-only); the "
+                                       "adjoint moves 29 arrays per iteration in two passes")
+
+
 def cpu_baseline(n, iters, threads):
     """The oracle (the reference's CPU kernels restated in C, OpenMP over the
     stencil loops) timed on the host cores at the same workload."""
@@ -198,6 +262,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--n-tts", type=int, default=127, help="time-to-tolerance grid (0: skip)")
     ap.add_argument("--cpu-3d", action="store_true", help="also time the oracle on the 3D block")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 127^2 and 2047^2 extra lines")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -222,6 +287,16 @@ def main():
         extra["tts_fp32_refined_vs_fp64"] = tts
         print(f"# TTS {args.n_tts}^3: FP64 {tts['fp64_s']} s, FP32+DC {tts['dc_s']} s "
               f"(ratio {tts['time_ratio_dc_over_fp64']})", file=sys.stderr, flush=True)
+    if not args.no_extra:
+        c5 = bench_c5()
+        extra["c5_forward_adjoint_2047"] = c5
+        print(f"# 2047^2 forward {c5['forward_ms_per_it']} ms/it, adjoint {c5['adjoint_ms_per_it']} ms/it",
+              file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            c1 = bench_c1(threads=min(16, os.cpu_count() or 1))
+            extra["c1_viscous_127_time_to_solution"] = c1
+            print(f"# 127^2 viscous: GPU {c1['gpu_s']} s, CPU {c1['cpu_s']} s ({c1['gpu_iters']} iterations)",
+                  file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu and not args.no_3d and args.cpu_3d:
         threads = min(16, os.cpu_count() or 1)

@@ -212,16 +212,18 @@ typedef struct pts_dc_result {
 
 int pts_dc_solve(pts_ctx* ctx, const pts_pt_params* p, const pts_dc_params* dp, pts_dc_result* res);
 
-/* ---- adjoint of the converged 2D viscous PT iteration --------------------
+/* ---- adjoint of the converged 2D PT iteration -------------------------------
  * No counterpart in the reference (the scripts have no adjoint).  Let Φ be one
  * PT iteration (compute_Pt_τ!, compute_V!, bc_y!(Vx), bc_x!(Vy) of
- * Stokes2D_ve_perf.jl:47-59,77-78 without the elastic terms), linear in the
- * state (Pt, τxx, τyy, τxy, Vx, Vy) for fixed μ, Gdτ, dτ_Rho.  For an objective
- * J(Vx, Vy) of the converged state the adjoint λ solves λ = Φᵀλ + ∂J/∂x; its
- * iteration contracts like the forward one, so no iteration history is kept.
- * Dirichlet (never-updated) velocity entries carry no adjoint.
- * Requires a 2D viscous context whose Mus, Gdτ, dτ_Rho are set up and whose
- * state is the converged forward solution (for pts_grad_logmu). */
+ * Stokes2D_ve_perf.jl:47-59,77-78; viscous: without the elastic terms), affine
+ * in the state (Pt, τxx, τyy, τxy, Vx, Vy) for fixed μ, Gdτ, dτ_Rho (and τ_o).
+ * For an objective J(Vx, Vy) of the converged state the adjoint λ solves
+ * λ = Φᵀλ + ∂J/∂x; its iteration contracts like the forward one, so no
+ * iteration history is kept.  Dirichlet (never-updated) velocity entries carry
+ * no adjoint.  Requires a 2D context whose Mus, Gdτ, dτ_Rho are set up and whose
+ * state is the converged forward solution (for pts_grad_logmu).  On a
+ * visco-elastic grid these act on the last physical step with τ_o fixed;
+ * pts_adjoint_ve_solve differentiates through all steps. */
 
 /* Set ∂J/∂Vx ((nx+1)*ny) and ∂J/∂Vy (nx*(ny+1)), host arrays, x fastest;
  * NULL = zero; entries on Dirichlet faces are ignored.  Resets λ to zero. */
@@ -238,6 +240,18 @@ int pts_adjoint_solve(pts_ctx* ctx, const pts_pt_params* p, const pts_solve_para
 /* ∂J/∂ln μ at the cell centres (μ = the Mus field), host array nx*ny, x
  * fastest, from the current λ and the converged forward stresses. */
 int pts_grad_logmu(pts_ctx* ctx, const pts_pt_params* p, double* grad, int64_t count);
+/* Visco-elastic time loop (no counterpart in the reference).  After the
+ * converged PT solve of physical step n (0-based; assign_τ! at the start of
+ * each step, Stokes2D_ve.jl:149-150) the caller stores its stresses with
+ * pts_checkpoint(ctx, n).  pts_adjoint_ve_solve then runs the adjoint
+ * backward over steps nsteps-1 .. 0 for J(V of the last step) (source set by
+ * pts_adjoint_set_source): each step's converged τ is the next step's τ_o, so
+ * step n's adjoint is driven by the τ_o-sensitivity of step n+1.  grad
+ * (host, nx*ny) receives ∂J/∂ln μ summed over the steps; res sums the
+ * iterations of the nsteps adjoint solves. */
+int pts_checkpoint(pts_ctx* ctx, int32_t slot);
+int pts_adjoint_ve_solve(pts_ctx* ctx, const pts_pt_params* p, int32_t nsteps, const pts_solve_params* sp,
+                         pts_solve_result* res, double* grad, int64_t count);
 
 #ifdef __cplusplus
 }

@@ -1,10 +1,11 @@
-// pts_adjoint.hip -- adjoint of the converged 2D viscous PT iteration and the
+// pts_adjoint.hip -- adjoint of the converged 2D PT iteration and the
 // gradient with respect to log viscosity (no counterpart in the reference).
 //
-// One forward PT iteration (multiply form, viscous) is the affine map
+// One forward PT iteration (multiply form) is the affine map
 //   Φ = B ∘ S_V ∘ S_Pτ :
-//   S_Pτ:  P' = P - r Gdτ ⊙ (D V),  τ' = a ⊙ τ + b ⊙ (E V),
-//          a = 1/(1 + Gdτ/μ), b = 2 Gdτ a (vertex means Ḡ, μ̄ for τxy)
+//   S_Pτ:  P' = P - r Gdτ ⊙ (D V),  τ' = a ⊙ (τ + Gr τ_o) + b ⊙ (E V),
+//          a = 1/(1 + Gdτ/μ + Gr), b = 2 Gdτ a, Gr = Gdτ/(G dt) (0 viscous)
+//          (vertex means of Gdτ and μ for τxy)
 //   S_V :  V_int' = V_int + Rh ⊙ (Dτ τ' - ∇P' [- ρg])   (Rh: face mean of dτ_Rho)
 //   B   :  free-slip copies V[:,0] = V[:,1], ... (bc_y!(Vx), bc_x!(Vy))
 // D is the discrete divergence, E the strain rate (ε̇xx, ε̇yy, ε̇xy), Dτ the
@@ -12,7 +13,9 @@
 // J(x*) is the fixed point of
 //   λ ← Φᵀ λ + ∂J/∂x = S_Pτᵀ S_Vᵀ Bᵀ λ + g,
 // which contracts at the rate of the forward iteration; no iteration history
-// is needed.  dJ/dm = (S_Vᵀ Bᵀ λ)_τ · ∂τ'/∂m.
+// is needed.  dJ/dm = (S_Vᵀ Bᵀ λ)_τ · ∂τ'/∂m.  Over visco-elastic physical
+// steps the converged τ of one step is the next step's τ_o, which adds
+// a·Gr ⊙ (S_Vᵀ Bᵀ λ_next)_τ to the previous step's source.
 //
 // pass 1 (Bᵀ, S_Vᵀ): fold the boundary-copy adjoints onto their sources,
 //   w = Rh ⊙ λV_int,  λP += D(w)-like divergence,  λτ += -Dτᵀ w.
@@ -93,35 +96,39 @@ __global__ __launch_bounds__(BX* BY) void k2_adj1(Fields f, Fields lam, Fields o
     }
 }
 
-__device__ __forceinline__ double acell(const Fields& f, int o) {
+// a = 1/(1 + Gdτ/μ + Gr) of the τ update, Gr = Gdτ/(G·dt) (visco-elastic;
+// igdt = 1/(G·dt)) or 0 (viscous, igdt = 0); b = 2·Gdτ·a.
+__device__ __forceinline__ double acell(const Fields& f, int o, double igdt) {
     const double g = f.p[PTS_GDTAU][o], m = f.p[PTS_MUS][o];
-    return 1.0 / (1.0 + g / m);
+    return 1.0 / (1.0 + g / m + g * igdt);
 }
 // b ⊙ λτ at cells (normal) and vertices (shear), zero off range
-__device__ __forceinline__ double pxx(const Fields& f, const Fields& l1, int i, int j) {
+__device__ __forceinline__ double pxx(const Fields& f, const Fields& l1, double igdt, int i, int j) {
     if (i < 0 || i >= f.nx || j < 0 || j >= f.ny) return 0.0;
     const int o = i + f.sx * j;
-    return 2.0 * f.p[PTS_GDTAU][o] * acell(f, o) * l1.p[PTS_TXX][o];
+    return 2.0 * f.p[PTS_GDTAU][o] * acell(f, o, igdt) * l1.p[PTS_TXX][o];
 }
-__device__ __forceinline__ double pyy(const Fields& f, const Fields& l1, int i, int j) {
+__device__ __forceinline__ double pyy(const Fields& f, const Fields& l1, double igdt, int i, int j) {
     if (i < 0 || i >= f.nx || j < 0 || j >= f.ny) return 0.0;
     const int o = i + f.sx * j;
-    return 2.0 * f.p[PTS_GDTAU][o] * acell(f, o) * l1.p[PTS_TYY][o];
+    return 2.0 * f.p[PTS_GDTAU][o] * acell(f, o, igdt) * l1.p[PTS_TYY][o];
 }
-__device__ __forceinline__ void vert_coef(const Fields& f, int o, double& a, double& b) {
+// vertex means of Gdτ and μ, and the vertex coefficients a, b
+__device__ __forceinline__ void vert_coef(const Fields& f, int o, double igdt, double& ag, double& am, double& a,
+                                          double& b) {
     const int sx = f.sx;
     const double* Gd = f.p[PTS_GDTAU];
     const double* Mu = f.p[PTS_MUS];
-    const double ag = 0.25 * (Gd[o] + Gd[o + 1] + Gd[o + sx] + Gd[o + sx + 1]);
-    const double am = 0.25 * (Mu[o] + Mu[o + 1] + Mu[o + sx] + Mu[o + sx + 1]);
-    a = 1.0 / (1.0 + ag / am);
+    ag = 0.25 * (Gd[o] + Gd[o + 1] + Gd[o + sx] + Gd[o + sx + 1]);
+    am = 0.25 * (Mu[o] + Mu[o + 1] + Mu[o + sx] + Mu[o + sx + 1]);
+    a = 1.0 / (1.0 + ag / am + ag * igdt);
     b = 2.0 * ag * a;
 }
-__device__ __forceinline__ double pxy(const Fields& f, const Fields& l1, int i, int j) {
+__device__ __forceinline__ double pxy(const Fields& f, const Fields& l1, double igdt, int i, int j) {
     if (i < 0 || i >= f.nx - 1 || j < 0 || j >= f.ny - 1) return 0.0;
     const int o = i + f.sx * j;
-    double a, b;
-    vert_coef(f, o, a, b);
+    double ag, am, a, b;
+    vert_coef(f, o, igdt, ag, am, a, b);
     return b * l1.p[PTS_TXY][o];
 }
 __device__ __forceinline__ double qP(const Fields& f, const Fields& l1, double r, int i, int j) {
@@ -130,31 +137,37 @@ __device__ __forceinline__ double qP(const Fields& f, const Fields& l1, double r
     return r * f.p[PTS_GDTAU][o] * l1.p[PTS_PT][o];
 }
 
-// pass 2: in = pass-1 output, out = new λ
-__global__ __launch_bounds__(BX* BY) void k2_adj2(Fields f, Fields l1, Fields out, StepScalars c, const double* gx,
-                                                  const double* gy) {
+// pass 2: in = pass-1 output, out = new λ; g holds the sources (∂J/∂x) under
+// the field ids VX, VY, TXX, TYY, TXY (null: none)
+__global__ __launch_bounds__(BX* BY) void k2_adj2(Fields f, Fields l1, Fields out, StepScalars c, double igdt,
+                                                  Fields g) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int nx = f.nx, ny = f.ny, sx = f.sx;
     if (i > nx || j > ny) return;
     const int o = i + sx * j;
     if (i < nx && j < ny) {
         out.p[PTS_PT][o] = l1.p[PTS_PT][o];
-        const double a = acell(f, o);
-        out.p[PTS_TXX][o] = a * l1.p[PTS_TXX][o];
-        out.p[PTS_TYY][o] = a * l1.p[PTS_TYY][o];
+        const double a = acell(f, o, igdt);
+        double txx = a * l1.p[PTS_TXX][o], tyy = a * l1.p[PTS_TYY][o];
+        if (g.p[PTS_TXX]) txx += g.p[PTS_TXX][o];
+        if (g.p[PTS_TYY]) tyy += g.p[PTS_TYY][o];
+        out.p[PTS_TXX][o] = txx;
+        out.p[PTS_TYY][o] = tyy;
     }
     if (i < nx - 1 && j < ny - 1) {
-        double a, b;
-        vert_coef(f, o, a, b);
-        out.p[PTS_TXY][o] = a * l1.p[PTS_TXY][o];
+        double ag, am, a, b;
+        vert_coef(f, o, igdt, ag, am, a, b);
+        double txy = a * l1.p[PTS_TXY][o];
+        if (g.p[PTS_TXY]) txy += g.p[PTS_TXY][o];
+        out.p[PTS_TXY][o] = txy;
     }
     if (j < ny) {  // Vx face (i, j); Dirichlet faces i = 0, nx carry no adjoint
         double v = 0.0;
         if (i >= 1 && i <= nx - 1) {
             v = l1.p[PTS_VX][o] - c._dx * (qP(f, l1, c.r, i - 1, j) - qP(f, l1, c.r, i, j)) +
-                c._dx * (pxx(f, l1, i - 1, j) - pxx(f, l1, i, j)) +
-                0.5 * c._dy * (pxy(f, l1, i - 1, j - 1) - pxy(f, l1, i - 1, j));
-            if (gx) v += gx[o];
+                c._dx * (pxx(f, l1, igdt, i - 1, j) - pxx(f, l1, igdt, i, j)) +
+                0.5 * c._dy * (pxy(f, l1, igdt, i - 1, j - 1) - pxy(f, l1, igdt, i - 1, j));
+            if (g.p[PTS_VX]) v += g.p[PTS_VX][o];
         }
         out.p[PTS_VX][o] = v;
     }
@@ -162,17 +175,19 @@ __global__ __launch_bounds__(BX* BY) void k2_adj2(Fields f, Fields l1, Fields ou
         double v = 0.0;
         if (j >= 1 && j <= ny - 1) {
             v = l1.p[PTS_VY][o] - c._dy * (qP(f, l1, c.r, i, j - 1) - qP(f, l1, c.r, i, j)) +
-                c._dy * (pyy(f, l1, i, j - 1) - pyy(f, l1, i, j)) +
-                0.5 * c._dx * (pxy(f, l1, i - 1, j - 1) - pxy(f, l1, i, j - 1));
-            if (gy) v += gy[o];
+                c._dy * (pyy(f, l1, igdt, i, j - 1) - pyy(f, l1, igdt, i, j)) +
+                0.5 * c._dx * (pxy(f, l1, igdt, i - 1, j - 1) - pxy(f, l1, igdt, i, j - 1));
+            if (g.p[PTS_VY]) v += g.p[PTS_VY][o];
         }
         out.p[PTS_VY][o] = v;
     }
 }
 
 // ∂J/∂ln μ at cells from the pass-1 adjoint of the converged λ and the
-// forward stresses τ* (= τ' at the fixed point).
-__global__ __launch_bounds__(BX* BY) void k2_adj_grad(Fields f, Fields l1, double* grad) {
+// converged stresses τ* (= τ' at the fixed point), added to grad when acc.
+// τ' = (τ + τ_o·Gr + 2Gdτ·ε̇)/D with D = 1 + Gdτ/μ + Gr gives
+// ∂τ'/∂ln μ = τ'·(Gdτ/μ)/D; the vertex mean of μ contributes μ/4 per cell.
+__global__ __launch_bounds__(BX* BY) void k2_adj_grad(Fields f, Fields l1, double igdt, double* grad, int acc) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int nx = f.nx, ny = f.ny, sx = f.sx;
     if (i >= nx || j >= ny) return;
@@ -180,18 +195,38 @@ __global__ __launch_bounds__(BX* BY) void k2_adj_grad(Fields f, Fields l1, doubl
     const double* Gd = f.p[PTS_GDTAU];
     const double* Mu = f.p[PTS_MUS];
     const double gm = Gd[o] / Mu[o];
-    double g = (l1.p[PTS_TXX][o] * f.p[PTS_TXX][o] + l1.p[PTS_TYY][o] * f.p[PTS_TYY][o]) * (gm / (1.0 + gm));
+    double g = (l1.p[PTS_TXX][o] * f.p[PTS_TXX][o] + l1.p[PTS_TYY][o] * f.p[PTS_TYY][o]) *
+               (gm / (1.0 + gm + Gd[o] * igdt));
     // shear: the (up to) four vertices around cell (i,j)
     for (int dj = -1; dj <= 0; ++dj)
         for (int di = -1; di <= 0; ++di) {
             const int vi = i + di, vj = j + dj;
             if (vi < 0 || vi >= nx - 1 || vj < 0 || vj >= ny - 1) continue;
             const int v = vi + sx * vj;
-            const double ag = 0.25 * (Gd[v] + Gd[v + 1] + Gd[v + sx] + Gd[v + sx + 1]);
-            const double am = 0.25 * (Mu[v] + Mu[v + 1] + Mu[v + sx] + Mu[v + sx + 1]);
-            g += l1.p[PTS_TXY][v] * f.p[PTS_TXY][v] * 0.25 * Mu[o] * (ag / (am * am)) / (1.0 + ag / am);
+            double ag, am, a, b;
+            vert_coef(f, v, igdt, ag, am, a, b);
+            g += l1.p[PTS_TXY][v] * f.p[PTS_TXY][v] * 0.25 * Mu[o] * (ag / (am * am)) * a;
         }
-    grad[i + (int64_t)nx * j] = g;
+    const int64_t q = i + (int64_t)nx * j;
+    grad[q] = acc ? grad[q] + g : g;
+}
+
+// The τ_o dependence of the previous physical step: its converged τ is this
+// step's τ_o, entering τ' with the factor a·Gr.  gτ = a·Gr ⊙ (pass-1 λ)_τ is
+// the previous step's adjoint source on τ.
+__global__ __launch_bounds__(BX* BY) void k2_adj_couple(Fields f, Fields l1, double igdt, Fields g) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int nx = f.nx, ny = f.ny, sx = f.sx;
+    if (i >= nx || j >= ny) return;
+    const int o = i + sx * j;
+    const double a = acell(f, o, igdt), Gr = f.p[PTS_GDTAU][o] * igdt;
+    g.p[PTS_TXX][o] = a * Gr * l1.p[PTS_TXX][o];
+    g.p[PTS_TYY][o] = a * Gr * l1.p[PTS_TYY][o];
+    if (i < nx - 1 && j < ny - 1) {
+        double ag, am, av, b;
+        vert_coef(f, o, igdt, ag, am, av, b);
+        g.p[PTS_TXY][o] = av * (ag * igdt) * l1.p[PTS_TXY][o];
+    }
 }
 
 // NaN-propagating max (fmax drops a NaN operand)
@@ -231,15 +266,28 @@ __global__ __launch_bounds__(256) void k2_adj_change(Fields f, Fields a, Fields 
     }
 }
 
-__global__ void k_max2_final(const double* part, int nb, double* out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_max2_final(const double* part, int nb, double* out) {
+    __shared__ double s0[256], s1[256];
+    const int t = threadIdx.x;
     double d = 0.0, m = 0.0;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = t; b < nb; b += 256) {
         d = nmax(d, part[2 * b]);
         m = nmax(m, part[2 * b + 1]);
     }
-    out[0] = d;
-    out[1] = m;
+    s0[t] = d;
+    s1[t] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            s0[t] = nmax(s0[t], s0[t + w]);
+            s1[t] = nmax(s1[t], s1[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = s0[0];
+        out[1] = s1[0];
+    }
 }
 
 inline dim3 grid2(int n1, int n2) { return dim3((n1 + BX - 1) / BX, (n2 + BY - 1) / BY, 1); }
@@ -247,9 +295,9 @@ inline dim3 grid2(int n1, int n2) { return dim3((n1 + BX - 1) / BX, (n2 + BY - 1
 }  // namespace
 
 hipError_t launch2_adj_step(hipStream_t s, const Fields& f, const Fields& lam, const Fields& tmp, const Fields& out,
-                            const StepScalars& c, const double* gx, const double* gy) {
+                            const StepScalars& c, double igdt, const Fields& g) {
     k2_adj1<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, lam, tmp, c);
-    k2_adj2<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, tmp, out, c, gx, gy);
+    k2_adj2<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, tmp, out, c, igdt, g);
     return hipGetLastError();
 }
 
@@ -259,15 +307,20 @@ hipError_t launch2_adj_pass1(hipStream_t s, const Fields& f, const Fields& lam, 
     return hipGetLastError();
 }
 
-hipError_t launch2_adj_grad(hipStream_t s, const Fields& f, const Fields& l1, double* grad) {
-    k2_adj_grad<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s>>>(f, l1, grad);
+hipError_t launch2_adj_grad(hipStream_t s, const Fields& f, const Fields& l1, double igdt, double* grad, bool acc) {
+    k2_adj_grad<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s>>>(f, l1, igdt, grad, acc ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch2_adj_couple(hipStream_t s, const Fields& f, const Fields& l1, double igdt, const Fields& g) {
+    k2_adj_couple<<<grid2(f.nx, f.ny), dim3(BX, BY), 0, s>>>(f, l1, igdt, g);
     return hipGetLastError();
 }
 
 hipError_t launch2_adj_change(hipStream_t s, const Fields& f, const Fields& a, const Fields& b, double* part,
                               double* out) {
     k2_adj_change<<<kRedBlocks, 256, 0, s>>>(f, a, b, part);
-    k_max2_final<<<1, 64, 0, s>>>(part, kRedBlocks, out);
+    k_max2_final<<<1, 256, 0, s>>>(part, kRedBlocks, out);
     return hipGetLastError();
 }
 

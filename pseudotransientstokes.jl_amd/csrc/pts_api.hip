@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <map>
+#include <vector>
 #include <string>
 
 #include "pts_internal.hpp"
@@ -49,9 +51,10 @@ struct pts_ctx {
     // source ∂J/∂Vx, ∂J/∂Vy and the gradient buffer
     double* adj[2][PTS_NFIELDS] = {};
     double* adj_tmp[PTS_NFIELDS] = {};
-    double* adj_src[2] = {};
+    double* adj_src[PTS_NFIELDS] = {};  // ∂J/∂x under VX, VY; step coupling under TXX, TYY, TXY
     double* adj_grad = nullptr;
     int adj_cur = 0;
+    std::vector<std::array<double*, 3>> ckpt;  // converged τxx, τyy, τxy of each physical step
     // Gdτ and dτ_Rho as produced by the last pts_iter_params from the current
     // Musτ: the single-pass kernels may then derive them from Musτ (one array
     // read instead of two).  Any write to Musτ, Gdτ or dτ_Rho clears dp_ok.
@@ -297,6 +300,8 @@ int pts_destroy(pts_ctx* c) {
     }
     for (double* p : c->adj_src) field_free(c, p);
     field_free(c, c->adj_grad);
+    for (auto& slot : c->ckpt)
+        for (double* p : slot) field_free(c, p);
     if (c->red_partials) (void)hipFree(c->red_partials);
     if (c->red_out) (void)hipFree(c->red_out);
     if (c->h_red) (void)hipHostFree(c->h_red);
@@ -987,8 +992,9 @@ int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pt
 }
 
 
-// ---- adjoint of the converged 2D viscous PT iteration ----------------------
+// ---- adjoint of the converged 2D PT iteration ---------------------------------
 static const int kAdj2[] = {PTS_PT, PTS_TXX, PTS_TYY, PTS_TXY, PTS_VX, PTS_VY};
+static const int kAdjSrc[] = {PTS_VX, PTS_VY, PTS_TXX, PTS_TYY, PTS_TXY};
 
 static int alloc_zero(pts_ctx* c, double** slot) {
     if (*slot) return PTS_OK;
@@ -1005,7 +1011,7 @@ static bool adj_field(int f) {
 }
 
 static int prepare_adj(pts_ctx* c) {
-    if (c->nd != 2 || c->g.ve) return fail(c, PTS_ERR_STATE, "the adjoint is implemented for 2D viscous grids");
+    if (c->nd != 2) return fail(c, PTS_ERR_STATE, "the adjoint is implemented for 2D grids");
     if (int rc = setdev(c)) return rc;
     if (int rc = ensure_list(c, {PTS_PT, PTS_VX, PTS_VY, PTS_TXX, PTS_TYY, PTS_TXY, PTS_MUS, PTS_GDTAU, PTS_DTAU_RHO}))
         return rc;
@@ -1014,8 +1020,8 @@ static int prepare_adj(pts_ctx* c) {
         if (int rc = alloc_zero(c, &c->adj[1][f])) return rc;
         if (int rc = alloc_zero(c, &c->adj_tmp[f])) return rc;
     }
-    for (double*& p : c->adj_src)
-        if (int rc = alloc_zero(c, &p)) return rc;
+    for (int f : kAdjSrc)
+        if (int rc = alloc_zero(c, &c->adj_src[f])) return rc;
     return alloc_zero(c, &c->adj_grad);
 }
 
@@ -1026,8 +1032,29 @@ static Fields adj_fields(const pts_ctx* c, double* const* src) {
     return f;
 }
 
+// sources of one adjoint solve: ∂J/∂V (with_v) and the coupling to the next
+// physical step (with_tau)
+static Fields src_fields(const pts_ctx* c, bool with_v, bool with_tau) {
+    Fields g = F(c);
+    for (int q = 0; q < PTS_NFIELDS; ++q) g.p[q] = nullptr;
+    if (with_v) g.p[PTS_VX] = c->adj_src[PTS_VX], g.p[PTS_VY] = c->adj_src[PTS_VY];
+    if (with_tau)
+        for (int q : {PTS_TXX, PTS_TYY, PTS_TXY}) g.p[q] = c->adj_src[q];
+    return g;
+}
+
 static StepScalars adj_scal(const pts_ctx* c, const pts_pt_params* p) {
     return scal(p->r, p->G, p->dt, p->dx, p->dy, 1.0);
+}
+
+// 1/(G·dt) of the visco-elastic τ update (0: viscous)
+static double adj_igdt(const pts_ctx* c, const pts_pt_params* p) { return c->g.ve ? 1.0 / (p->G * p->dt) : 0.0; }
+
+static int zero_lambda(pts_ctx* c) {
+    for (int f : kAdj2)
+        for (int b = 0; b < 2; ++b)
+            HIPCHK(c, hipMemsetAsync(c->adj[b][f], 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    return PTS_OK;
 }
 
 int pts_adjoint_set_source(pts_ctx* c, const double* dJ_dVx, int64_t nvx, const double* dJ_dVy, int64_t nvy) {
@@ -1037,13 +1064,11 @@ int pts_adjoint_set_source(pts_ctx* c, const double* dJ_dVx, int64_t nvx, const 
     const double* hs[2] = {dJ_dVx, dJ_dVy};
     const int64_t ns[2] = {nvx, nvy};
     for (int q = 0; q < 2; ++q) {
-        HIPCHK(c, hipMemsetAsync(c->adj_src[q], 0, (size_t)c->nalloc * sizeof(double), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->adj_src[fs[q]], 0, (size_t)c->nalloc * sizeof(double), c->stream));
         if (hs[q])
-            if (int rc = copy_dense(c, fs[q], c->adj_src[q], nullptr, hs[q], ns[q], true)) return rc;
+            if (int rc = copy_dense(c, fs[q], c->adj_src[fs[q]], nullptr, hs[q], ns[q], true)) return rc;
     }
-    for (int f : kAdj2)
-        for (int b = 0; b < 2; ++b)
-            HIPCHK(c, hipMemsetAsync(c->adj[b][f], 0, (size_t)c->nalloc * sizeof(double), c->stream));
+    if (int rc = zero_lambda(c)) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PTS_OK;
 }
@@ -1062,12 +1087,12 @@ int pts_adjoint_download(pts_ctx* c, int f, double* host, int64_t count) {
     return copy_dense(c, f, c->adj[c->adj_cur][f], host, nullptr, count, false);
 }
 
-static int adj_steps(pts_ctx* c, const StepScalars& s, int64_t n) {
+static int adj_steps(pts_ctx* c, const StepScalars& s, double igdt, const Fields& g, int64_t n) {
     const Fields f = F(c), t = adj_fields(c, c->adj_tmp);
     for (int64_t q = 0; q < n; ++q) {
         const int a = c->adj_cur;
-        HIPCHK(c, launch2_adj_step(c->stream, f, adj_fields(c, c->adj[a]), t, adj_fields(c, c->adj[a ^ 1]), s,
-                                   c->adj_src[0], c->adj_src[1]));
+        HIPCHK(c, launch2_adj_step(c->stream, f, adj_fields(c, c->adj[a]), t, adj_fields(c, c->adj[a ^ 1]), s, igdt,
+                                   g));
         c->adj_cur = a ^ 1;
     }
     return PTS_OK;
@@ -1076,23 +1101,20 @@ static int adj_steps(pts_ctx* c, const StepScalars& s, int64_t n) {
 int pts_adjoint_iterate(pts_ctx* c, const pts_pt_params* p, int64_t niter) {
     if (!c || !p || niter < 0) return fail(c, PTS_ERR_ARG, "bad argument");
     if (int rc = prepare_adj(c)) return rc;
-    if (int rc = adj_steps(c, adj_scal(c, p), niter)) return rc;
+    if (int rc = adj_steps(c, adj_scal(c, p), adj_igdt(c, p), src_fields(c, true, false), niter)) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PTS_OK;
 }
 
-int pts_adjoint_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp, pts_solve_result* res) {
-    if (!c || !p || !sp || !res || sp->nout < 1) return fail(c, PTS_ERR_ARG, "bad argument");
-    if (int rc = prepare_adj(c)) return rc;
-    const StepScalars s = adj_scal(c, p);
-    std::memset(res, 0, sizeof(*res));
+// λ <- Φᵀλ + g until max|Δλ_V| <= eps·max|λ_V|, checked every nout iterations
+static int adj_solve_loop(pts_ctx* c, const StepScalars& s, double igdt, const Fields& g, const pts_solve_params* sp,
+                          pts_solve_result* res) {
     int64_t iter = 0;
     double err = INFINITY;
     while (err > sp->eps && iter < sp->iter_max) {
         const int64_t n = std::min<int64_t>(sp->nout, sp->iter_max - iter);
-        if (int rc = adj_steps(c, s, n)) return rc;
+        if (int rc = adj_steps(c, s, igdt, g, n)) return rc;
         iter += n;
-        // change of λV over the last iteration relative to max |λV|
         HIPCHK(c, launch2_adj_change(c->stream, F(c), adj_fields(c, c->adj[c->adj_cur]),
                                      adj_fields(c, c->adj[c->adj_cur ^ 1]), c->red_partials, c->red_out));
         HIPCHK(c, hipMemcpyAsync(c->h_red, c->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1102,15 +1124,24 @@ int pts_adjoint_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params
         res->comp[1] = m;
         res->nchecks += 1;
         if (d != d || m != m) {
-            res->iters = iter;
+            res->iters += iter;
             res->err = NAN;
             return fail(c, PTS_ERR_NAN, "NaN in the adjoint state after " + std::to_string(iter) + " iterations");
         }
         err = m > 0.0 ? d / m : 0.0;
     }
-    res->iters = iter;
-    res->err = err;
-    res->converged = err <= sp->eps ? 1 : 0;
+    res->iters += iter;
+    res->err = std::max(res->err, err);
+    if (err > sp->eps) res->converged = 0;
+    return PTS_OK;
+}
+
+int pts_adjoint_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp, pts_solve_result* res) {
+    if (!c || !p || !sp || !res || sp->nout < 1) return fail(c, PTS_ERR_ARG, "bad argument");
+    if (int rc = prepare_adj(c)) return rc;
+    std::memset(res, 0, sizeof(*res));
+    res->converged = 1;
+    if (int rc = adj_solve_loop(c, adj_scal(c, p), adj_igdt(c, p), src_fields(c, true, false), sp, res)) return rc;
     if (!res->converged) return fail(c, PTS_ERR_NOCONV, "adjoint: iter_max reached with err > eps");
     return PTS_OK;
 }
@@ -1122,9 +1153,57 @@ int pts_grad_logmu(pts_ctx* c, const pts_pt_params* p, double* grad, int64_t cou
         return fail(c, PTS_ERR_ARG, "count " + std::to_string(count) + " != nx*ny");
     const Fields t = adj_fields(c, c->adj_tmp);
     HIPCHK(c, launch2_adj_pass1(c->stream, F(c), adj_fields(c, c->adj[c->adj_cur]), t, adj_scal(c, p)));
-    HIPCHK(c, launch2_adj_grad(c->stream, F(c), t, c->adj_grad));
+    HIPCHK(c, launch2_adj_grad(c->stream, F(c), t, adj_igdt(c, p), c->adj_grad, false));
     HIPCHK(c, hipMemcpyAsync(grad, c->adj_grad, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PTS_OK;
+}
+
+int pts_checkpoint(pts_ctx* c, int32_t slot) {
+    if (!c || slot < 0 || slot > 100000) return fail(c, PTS_ERR_ARG, "bad checkpoint slot");
+    if (c->nd != 2) return fail(c, PTS_ERR_STATE, "checkpoints are implemented for 2D grids");
+    if (int rc = setdev(c)) return rc;
+    if (int rc = ensure_list(c, {PTS_TXX, PTS_TYY, PTS_TXY})) return rc;
+    if ((int)c->ckpt.size() <= slot) c->ckpt.resize(slot + 1, std::array<double*, 3>{nullptr, nullptr, nullptr});
+    const int fs[3] = {PTS_TXX, PTS_TYY, PTS_TXY};
+    for (int q = 0; q < 3; ++q) {
+        if (int rc = alloc_zero(c, &c->ckpt[slot][q])) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->ckpt[slot][q], c->cur[fs[q]], (size_t)c->nalloc * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+    }
+    return PTS_OK;
+}
+
+int pts_adjoint_ve_solve(pts_ctx* c, const pts_pt_params* p, int32_t nsteps, const pts_solve_params* sp,
+                         pts_solve_result* res, double* grad, int64_t count) {
+    if (!c || !p || !sp || !res || !grad || sp->nout < 1 || nsteps < 1) return fail(c, PTS_ERR_ARG, "bad argument");
+    if (!c->g.ve) return fail(c, PTS_ERR_STATE, "pts_adjoint_ve_solve needs a visco-elastic grid");
+    if (count != (int64_t)c->nx * c->ny) return fail(c, PTS_ERR_ARG, "count " + std::to_string(count) + " != nx*ny");
+    if ((int)c->ckpt.size() < nsteps) return fail(c, PTS_ERR_STATE, "missing checkpoints");
+    for (int n = 0; n < nsteps; ++n)
+        if (!c->ckpt[n][0]) return fail(c, PTS_ERR_STATE, "missing checkpoint " + std::to_string(n));
+    if (int rc = prepare_adj(c)) return rc;
+    std::memset(res, 0, sizeof(*res));
+    res->converged = 1;
+    const StepScalars s = adj_scal(c, p);
+    const double igdt = adj_igdt(c, p);
+    const Fields t = adj_fields(c, c->adj_tmp);
+    for (int n = nsteps - 1; n >= 0; --n) {
+        // step n: λ = Φᵀλ + ∂J/∂V (last step) + coupling from step n+1
+        if (int rc = zero_lambda(c)) return rc;
+        const Fields g = src_fields(c, n == nsteps - 1, n < nsteps - 1);
+        if (int rc = adj_solve_loop(c, s, igdt, g, sp, res)) return rc;
+        HIPCHK(c, launch2_adj_pass1(c->stream, F(c), adj_fields(c, c->adj[c->adj_cur]), t, s));
+        Fields fk = F(c);  // the converged stresses of step n
+        fk.p[PTS_TXX] = c->ckpt[n][0];
+        fk.p[PTS_TYY] = c->ckpt[n][1];
+        fk.p[PTS_TXY] = c->ckpt[n][2];
+        HIPCHK(c, launch2_adj_grad(c->stream, fk, t, igdt, c->adj_grad, n < nsteps - 1));
+        if (n > 0) HIPCHK(c, launch2_adj_couple(c->stream, F(c), t, igdt, src_fields(c, false, true)));
+    }
+    HIPCHK(c, hipMemcpyAsync(grad, c->adj_grad, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!res->converged) return fail(c, PTS_ERR_NOCONV, "adjoint: iter_max reached with err > eps");
     return PTS_OK;
 }
 

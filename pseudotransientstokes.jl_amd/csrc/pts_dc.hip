@@ -135,7 +135,7 @@ __global__ __launch_bounds__(64) void k2_dc(const Fields32 in, const Fields32 ou
 // marching in z), viscous, multiply form, with sources.
 // ---------------------------------------------------------------------------
 constexpr int BY3 = 12;
-__global__ __launch_bounds__(64 * BY3) void k3_dc(const Fields32 in, const Fields32 out, const StepScalars32 cs,
+__global__ __launch_bounds__(64 * BY3, 6) void k3_dc(const Fields32 in, const Fields32 out, const StepScalars32 cs,
                                                   int Lz, int ntx, int nty) {
     __shared__ float sh[4][BY3][64];
     const int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -429,6 +429,9 @@ __global__ __launch_bounds__(256) void k_to_f32(float* dst, const double* src, i
 // Correction-problem residual norms (FP32 data, FP64 accumulation):
 // ∇·δV + R_P and ∇·δτ - ∇δP + R_V; out[0..3] = Σ², out[4..7] = max|.|
 // for Rx, Ry, Rz, ∇V.
+// NaN-propagating max (fmax drops a NaN operand)
+__device__ __forceinline__ double nanmax(double a, double b) { return (a != a) ? a : ((b != b) ? b : fmax(a, b)); }
+
 template <int ND>
 __global__ __launch_bounds__(256) void k_dc_norms(Fields32 g, StepScalars32 c, double* partials) {
     __shared__ double sm[8][256];
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(256) void k_dc_norms(Fields32 g, StepScalars32 c, d
     const float* Pt = g.p[PTS_PT];
     auto acc = [&](int comp, double v) {
         a[comp] += v * v;
-        a[4 + comp] = fmax(a[4 + comp], fabs(v));
+        a[4 + comp] = nanmax(a[4 + comp], fabs(v));
     };
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(q % n1);
@@ -488,21 +491,33 @@ __global__ __launch_bounds__(256) void k_dc_norms(Fields32 g, StepScalars32 c, d
     for (int w = 128; w > 0; w >>= 1) {
         if (t < w) {
             for (int m = 0; m < 4; ++m) sm[m][t] += sm[m][t + w];
-            for (int m = 4; m < 8; ++m) sm[m][t] = fmax(sm[m][t], sm[m][t + w]);
+            for (int m = 4; m < 8; ++m) sm[m][t] = nanmax(sm[m][t], sm[m][t + w]);
         }
         __syncthreads();
     }
     if (t < 8) partials[(int64_t)blockIdx.x * 8 + t] = sm[t][0];
 }
 
-__global__ void k_dc_norms_final(const double* partials, int nb, double* out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// 256 threads: each folds every 256th block, then a fixed-order tree
+// (deterministic sums; NaN-propagating maxima).
+__global__ __launch_bounds__(256) void k_dc_norms_final(const double* partials, int nb, double* out) {
+    __shared__ double sm[8][256];
+    const int t = threadIdx.x;
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < nb; ++b) {
+    for (int b = t; b < nb; b += 256) {
         for (int m = 0; m < 4; ++m) a[m] += partials[(int64_t)b * 8 + m];
-        for (int m = 4; m < 8; ++m) a[m] = fmax(a[m], partials[(int64_t)b * 8 + m]);
+        for (int m = 4; m < 8; ++m) a[m] = nanmax(a[m], partials[(int64_t)b * 8 + m]);
     }
-    for (int m = 0; m < 8; ++m) out[m] = a[m];
+    for (int m = 0; m < 8; ++m) sm[m][t] = a[m];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int m = 0; m < 4; ++m) sm[m][t] += sm[m][t + w];
+            for (int m = 4; m < 8; ++m) sm[m][t] = nanmax(sm[m][t], sm[m][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t < 8) out[t] = sm[t][0];
 }
 
 inline int gridN(int64_t tot) { return (int)std::min<int64_t>(65536, std::max<int64_t>(1, (tot + 255) / 256)); }
@@ -538,7 +553,7 @@ hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out
         k2_dc<<<nb, 64, 0, s>>>(in, out, c, L, ns);
     } else {
         const int ntx = (in.nx + 1 + 61) / 62, nty = (in.ny + 1 + BY3 - 3) / (BY3 - 2);
-        const int Lz = march_length(in.nz, ntx * nty, device_concurrency(k3_dc, 64 * BY3, conc3), 64);
+        const int Lz = march_length(in.nz, ntx * nty, device_concurrency(k3_dc, 64 * BY3, conc3), 24);
         const int nch = (in.nz + Lz - 1) / Lz;
         k3_dc<<<ntx * nty * nch, dim3(64, BY3), 0, s>>>(in, out, c, Lz, ntx, nty);
     }
@@ -576,7 +591,7 @@ hipError_t launch_dc_norms(hipStream_t s, const Fields32& g, const StepScalars32
                            double* out) {
     if (ndim == 2) k_dc_norms<2><<<kRedBlocks, 256, 0, s>>>(g, c, partials);
     else k_dc_norms<3><<<kRedBlocks, 256, 0, s>>>(g, c, partials);
-    k_dc_norms_final<<<1, 64, 0, s>>>(partials, kRedBlocks, out);
+    k_dc_norms_final<<<1, 256, 0, s>>>(partials, kRedBlocks, out);
     return hipGetLastError();
 }
 

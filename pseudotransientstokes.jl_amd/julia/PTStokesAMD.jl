@@ -160,4 +160,19 @@ function grad_logμ(c::Ctx, p::PtParams, nx, ny)
     return g
 end
 
+# visco-elastic time loop: checkpoint each step's converged τ, then the
+# backward pass over all steps (τ of step n is τ_o of step n+1)
+checkpoint!(c::Ctx, slot) = chk(c, ccall((:pts_checkpoint, libpts), Cint, (Ptr{Cvoid}, Int32), c.h, slot))
+function adjoint_ve_solve!(c::Ctx, p::PtParams, nsteps, ε, nout, iterMax, nx, ny)
+    sp = Ref(SolveParams(ε, nout, iterMax, 0, 0))
+    res = Ref(SolveResult())
+    g = zeros(Float64, nx, ny)
+    rc = ccall((:pts_adjoint_ve_solve, libpts), Cint,
+               (Ptr{Cvoid}, Ref{PtParams}, Int32, Ref{SolveParams}, Ref{SolveResult}, Ptr{Float64}, Int64),
+               c.h, Ref(p), nsteps, sp, res, g, length(g))
+    rc == -3 && error("NaN")
+    rc in (0, -4) || chk(c, rc)
+    return g, res[]
+end
+
 end # module

@@ -130,6 +130,9 @@ SIGNATURES = [
     ("pts_adjoint_solve", _i, [_vp, ctypes.POINTER(PtParams), ctypes.POINTER(SolveParams),
                                ctypes.POINTER(SolveResult)]),
     ("pts_grad_logmu", _i, [_vp, ctypes.POINTER(PtParams), _dp, _i64]),
+    ("pts_checkpoint", _i, [_vp, _i32]),
+    ("pts_adjoint_ve_solve", _i, [_vp, ctypes.POINTER(PtParams), _i32, ctypes.POINTER(SolveParams),
+                                  ctypes.POINTER(SolveResult), _dp, _i64]),
 ]
 
 _lib = None

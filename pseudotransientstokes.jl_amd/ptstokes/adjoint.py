@@ -1,19 +1,22 @@
 """Gradients of a velocity misfit with respect to log viscosity, by the
-adjoint of the converged 2D viscous PT iteration (no reference counterpart:
-the reference scripts have no adjoint).
+adjoint of the converged 2D PT iteration (no reference counterpart: the
+reference scripts have no adjoint).
 
-The forward model is the 2D viscous multiply-form iteration of
-scripts/Stokes2D_ve_perf.jl:47-59 without the elastic terms, with the
-free-slip copies of :77-78, driven to convergence on the device.  The
-objective is the misfit of the tangential velocity at ``npts`` seeded points
-of the top boundary row (Vx[I, ny-1], the free-slip copy of the top interior
-row):
+The forward model is the 2D multiply-form iteration of
+scripts/Stokes2D_ve_perf.jl:47-59 (viscous: without the elastic terms), with
+the free-slip copies of :77-78, driven to convergence on the device; the
+visco-elastic model runs ``nt`` physical steps (assign_τ! then the PT loop,
+Stokes2D_ve.jl:149-170) and stores each step's converged stresses as a
+checkpoint for the backward pass.  The objective is the misfit of the
+tangential velocity (after the last step) at ``npts`` seeded points of the
+top boundary row (Vx[I, ny-1], the free-slip copy of the top interior row):
 
     J(ln μ) = ½ Σ_k (Vx[I_k, ny-1] − obs_k)²
 
 ``Inversion.value_and_grad`` solves the forward problem, sets ∂J/∂Vx, runs the
-adjoint PT solve (libpts ``pts_adjoint_solve``) and returns ∂J/∂ln μ at the
-cell centres (``pts_grad_logmu``).  ``rrule`` packages the pair the way a
+adjoint PT solve (libpts ``pts_adjoint_solve``, or ``pts_adjoint_ve_solve``
+backward over the checkpointed steps) and returns ∂J/∂ln μ at the cell
+centres (``pts_grad_logmu``).  ``rrule`` packages the pair the way a
 ChainRulesCore rule does: the value and a pullback.
 """
 import numpy as np
@@ -30,14 +33,15 @@ def observation_points(nx, npts=256, seed=7):
 
 
 class Inversion:
-    """Forward solve, misfit and adjoint gradient for one 2D viscous grid."""
+    """Forward solve, misfit and adjoint gradient for one 2D grid (viscous,
+    or visco-elastic over the preset's nt physical steps)."""
 
     def __init__(self, n=255, npts=256, obs_seed=7, eps=1e-10, adj_eps=1e-10, nout=200, iter_max=10 ** 7,
                  preset="Stokes2D_grf", device=0, **overrides):
         self.model = StokesModel(preset, n=n, device=device, **overrides)
         p = self.model.p
-        if p["ndim"] != 2 or p["ve"]:
-            raise ValueError("the adjoint is implemented for 2D viscous presets")
+        if p["ndim"] != 2:
+            raise ValueError("the adjoint is implemented for 2D presets")
         self.model.setup()
         self.ctx = self.model.ctx
         self.params = self.model.pt_params()
@@ -78,11 +82,22 @@ class Inversion:
     # -- forward ----------------------------------------------------------
     def solve_forward(self):
         p = self.model.p
-        res = self.ctx.pt_solve(self.params, self.eps, self.nout, self.iter_max, p["norm_kind"])
-        if not res["converged"]:
-            raise RuntimeError(f"forward PT solve did not converge: {res}")
-        self.last["forward"] = res
-        return res
+        if not p["ve"]:
+            res = self.ctx.pt_solve(self.params, self.eps, self.nout, self.iter_max, p["norm_kind"])
+            if not res["converged"]:
+                raise RuntimeError(f"forward PT solve did not converge: {res}")
+            self.last["forward"] = res
+            return res
+        iters = 0
+        for n in range(p["nt"]):
+            self.ctx.assign_tau()
+            res = self.ctx.pt_solve(self.params, self.eps, self.nout, self.iter_max, p["norm_kind"])
+            if not res["converged"]:
+                raise RuntimeError(f"forward PT solve of step {n} did not converge: {res}")
+            iters += res["iters"]
+            self.ctx.checkpoint(n)
+        self.last["forward"] = dict(iters=iters, steps=p["nt"], err=res["err"], converged=True)
+        return self.last["forward"]
 
     def top_velocity(self):
         return self.ctx.download("Vx")[self.points, self.ny - 1]
@@ -108,14 +123,20 @@ class Inversion:
         return g, 0.5 * float(np.dot(r, r))
 
     def gradient(self):
-        """∂J/∂ln μ at the converged forward state now on the device."""
+        """∂J/∂ln μ at the converged forward state now on the device (for the
+        visco-elastic model: through all checkpointed steps)."""
         g, _ = self.dJ_dVx()
         self.ctx.adjoint_set_source(g, None)
-        res = self.ctx.adjoint_solve(self.params, self.adj_eps, self.nout, self.iter_max)
+        if self.model.p["ve"]:
+            grad, res = self.ctx.adjoint_ve_solve(self.params, self.model.p["nt"], self.adj_eps, self.nout,
+                                                  self.iter_max)
+        else:
+            res = self.ctx.adjoint_solve(self.params, self.adj_eps, self.nout, self.iter_max)
+            grad = None
         if not res["converged"]:
             raise RuntimeError(f"adjoint PT solve did not converge: {res}")
         self.last["adjoint"] = res
-        return self.ctx.grad_logmu(self.params)
+        return grad if grad is not None else self.ctx.grad_logmu(self.params)
 
     def value_and_grad(self, logmu):
         self.set_logmu(logmu)

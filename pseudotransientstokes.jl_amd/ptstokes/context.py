@@ -251,3 +251,21 @@ class Context:
         out = np.empty(self.nx * self.ny, dtype=np.float64)
         self._chk(self.lib.pts_grad_logmu(self.h, ctypes.byref(params), out.ctypes.data_as(_lib._dp), out.size))
         return out.reshape((self.nx, self.ny), order="F")
+
+    def checkpoint(self, slot):
+        """Store the current stresses (the converged state of physical step slot)."""
+        self._chk(self.lib.pts_checkpoint(self.h, int(slot)))
+
+    def adjoint_ve_solve(self, params, nsteps, eps, nout, iter_max):
+        """Adjoint of nsteps checkpointed visco-elastic steps; returns
+        (∂J/∂ln μ of shape (nx, ny), result dict)."""
+        sp = _lib.SolveParams(eps, int(nout), int(iter_max), 0, 0)
+        res = _lib.SolveResult()
+        g = np.empty(self.nx * self.ny, dtype=np.float64)
+        rc = self.lib.pts_adjoint_ve_solve(self.h, ctypes.byref(params), int(nsteps), ctypes.byref(sp),
+                                           ctypes.byref(res), g.ctypes.data_as(_lib._dp), g.size)
+        out = dict(iters=res.iters, err=res.err, converged=bool(res.converged), nchecks=res.nchecks)
+        if rc != _lib.PTS_ERR_NOCONV:
+            self._chk(rc)
+        return g.reshape((self.nx, self.ny), order="F"), out
+

@@ -55,6 +55,11 @@ PRESETS["Stokes3D_grf"] = dict(_COMMON, ndim=3, ve=False, fd="mul", nt=1, iter_m
 PRESETS["Stokes2D_grf"] = dict(_COMMON, ndim=2, ve=False, fd="mul", nt=1, iter_max=10 ** 6, nout=500,
                                CFL=0.9 / math.sqrt(2), Re=5 * math.pi, r=1.0, G=1.0, dt=1.0, n=127,
                                norm_kind=1, grf_seed=1234, rad2=0.0)
+# Visco-elastic inversion: the same field, G = 1, dt = η/G/10 with η = 1,
+# 10 physical steps (Stokes2D_ve_perf.jl numerics).
+PRESETS["Stokes2D_ve_grf"] = dict(_COMMON, ndim=2, ve=True, fd="mul", nt=10, iter_max=10 ** 6, nout=500,
+                                  CFL=0.8 / math.sqrt(2), Re=5 * math.pi, r=1.0, G=1.0, dt=0.1, n=127,
+                                  norm_kind=1, grf_seed=1234, rad2=0.0)
 
 # 1D dispersion optimum (dispersion_analysis/dispersion_analysis_stokes1D.ipynb
 # cell 5 output): r = 1/4, Re = 9*sqrt(3)*pi/4

@@ -52,12 +52,17 @@ def build_matrix(o, shapes):
     return np.array(rows).T, index  # M[i, j] = ∂out_i/∂in_j
 
 
-@pytest.mark.parametrize("nx,ny", [(9, 7), (6, 10)])
-def test_adjoint_step_is_transpose_of_oracle_iteration(nx, ny):
+@pytest.mark.parametrize("nx,ny,ve", [(9, 7, 0), (6, 10, 0), (8, 7, 1), (7, 9, 1)])
+def test_adjoint_step_is_transpose_of_oracle_iteration(nx, ny, ve):
+    # visco-elastic: τ_o = 0 (it is a parameter of the step, not state), G·dt = 0.35
     rng = np.random.default_rng(nx * 100 + ny)
-    kw = dict(nx=nx, ny=ny, perf=1, ve=0, ebg=0.0, rhog_in=0.0, rhog_out=0.0, rad2=0.0)
+    G, dt = 0.7, 0.5
+    kw = dict(nx=nx, ny=ny, perf=1, ve=ve, ebg=0.0, rhog_in=0.0, rhog_out=0.0, rad2=0.0, G=G, dt=dt)
     o = Oracle("Stokes2D", **kw).setup()
-    c = Context(2, nx, ny, ve=False, fd_mode="mul")
+    c = Context(2, nx, ny, ve=bool(ve), fd_mode="mul")
+    if ve:
+        for q in ("txx_o", "tyy_o", "txy_o"):
+            o.set(q, np.zeros(o.shape(q)))
     dx, dy = 10.0 / nx, 10.0 / ny
     coef = {"Mus": np.exp(rng.normal(0.0, 1.5, (nx, ny))),
             "Gdtau": rng.uniform(0.2, 2.0, (nx, ny)),
@@ -72,7 +77,7 @@ def test_adjoint_step_is_transpose_of_oracle_iteration(nx, ny):
         o.set(q, np.zeros(shapes[q]))
     o.iterate(1)
     assert all(np.all(o.get(q) == 0.0) for q in ADJ)
-    params = Context.pt_params(1.0, 1.0, 1.0, dx, dy)
+    params = Context.pt_params(1.0, G, dt, dx, dy)
     c.adjoint_set_source(None, None)
     for trial in range(3):
         lam = {n: rng.normal(size=shapes[n]) * state_mask(n, shapes[n]) for n in ADJ}
@@ -240,3 +245,45 @@ def test_rrule_pullback_and_points():
     assert Jt == 0.0 and np.all(gt == 0.0)
     inv.close()
     inv2.close()
+
+
+def test_ve_gradient_through_checkpointed_steps():
+    """Visco-elastic, 4 physical steps (each step's τ is the next step's τ_o):
+    the checkpointed backward pass against central differences of the whole
+    forward time loop (device forward, bit-identical to the oracle's; the
+    oracle reproduces J at the base point)."""
+    n, h, feps, nt = 48, 1e-5, 1e-12, 4
+    inv = Inversion(n=n, preset="Stokes2D_ve_grf", nt=nt, eps=feps, adj_eps=1e-14, nout=100, npts=40)
+    inv.observe(inv.grf_logmu(4321))
+    m0 = inv.grf_logmu(1234)
+    J0, grad = inv.value_and_grad(m0)
+    assert inv.last["adjoint"]["converged"] and np.all(np.isfinite(grad))
+    kw = oracle_kwargs(inv.model)
+    kw.update(eps=feps, nout=100, iter_max=10 ** 6)
+    inv.set_logmu(m0)
+    o = Oracle("Stokes2D", **kw).setup()
+    for f in ("Mus", "Gdtau", "dtau_Rho", "Mustau"):
+        o.set(f, inv.ctx.download(f))
+    for f in ("Pt", "Vx", "Vy", "txx", "tyy", "txy", "txx_o", "tyy_o", "txy_o"):
+        o.set(f, inv.ctx.download(f))
+    r = o.solve()
+    assert r["rc"] == 0
+    v = o.get("Vx")[inv.points, n - 1] - inv.obs
+    assert abs(0.5 * float(np.dot(v, v)) - J0) <= 1e-9 * abs(J0)
+    fds, ads = [], []
+    for d in fd_directions(m0.shape, 6, 11):
+        fds.append((inv.value(m0 + h * d) - inv.value(m0 - h * d)) / (2 * h))
+        ads.append(float(np.sum(grad * d)))
+    _check_fd(fds, ads, 1e-5, 1e-4)
+    # one step only (τ_o fixed) would miss the coupling: the multi-step
+    # gradient is not the last step's
+    inv.set_logmu(m0)
+    inv.solve_forward()
+    g1, _ = inv.dJ_dVx()
+    inv.ctx.adjoint_set_source(g1, None)
+    res = inv.ctx.adjoint_solve(inv.params, 1e-14, 100, 10 ** 7)
+    assert res["converged"]
+    last = inv.ctx.grad_logmu(inv.params)
+    assert np.max(np.abs(last - grad)) > 1e-3 * np.max(np.abs(grad))
+    inv.close()
+
