@@ -103,7 +103,7 @@ def bench_3d(n, steps, warmup, ev):
     return out
 
 
-def bench_tts(n, nout=500, rtol_inner=1e-4):
+def bench_tts(n, nout=500, rtol_inner=1e-4, use_graph=True):
     """Time to ‖res‖∞ < 1e-8 on the 3D viscous random-viscosity problem
     (log10 μ a seeded Gaussian random field clipped to ±2, 1e-3 sphere with
     ρg_z = -1): FP64 PT against FP32 PT sweeps inside FP64 defect
@@ -115,7 +115,7 @@ def bench_tts(n, nout=500, rtol_inner=1e-4):
     m = StokesModel("Stokes3D_grf", n=n, nout=nout).setup()
     m.ctx.sync()
     t0 = time.perf_counter()
-    r64 = m.step()
+    r64 = m.step(use_graph=use_graph)
     m.ctx.sync()
     t64 = time.perf_counter() - t0
     ref = {f: m.get(f) for f in names}
@@ -263,6 +263,8 @@ def main():
     ap.add_argument("--n-tts", type=int, default=127, help="time-to-tolerance grid (0: skip)")
     ap.add_argument("--cpu-3d", action="store_true", help="also time the oracle on the 3D block")
     ap.add_argument("--no-extra", action="store_true", help="skip the 127^2 and 2047^2 extra lines")
+    ap.add_argument("--tts-no-graph", action="store_true",
+                    help="FP64 time-to-tolerance without HIP graphs (for tracing tools)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -283,7 +285,7 @@ def main():
         print(f"# 3D VE {args.n3d}^3: {r3['ms_per_it']:.4f} ms/it, T_eff {r3['T_eff_GBs']:.1f} GB/s",
               file=sys.stderr, flush=True)
     if args.n_tts > 0:
-        tts = bench_tts(args.n_tts)
+        tts = bench_tts(args.n_tts, use_graph=not args.tts_no_graph)
         extra["tts_fp32_refined_vs_fp64"] = tts
         print(f"# TTS {args.n_tts}^3: FP64 {tts['fp64_s']} s, FP32+DC {tts['dc_s']} s "
               f"(ratio {tts['time_ratio_dc_over_fp64']})", file=sys.stderr, flush=True)
