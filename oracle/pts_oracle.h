@@ -39,7 +39,8 @@ typedef struct or_params {
     int32_t nt, nout;
     int64_t iter_max;
     int32_t nsmooth;   /* smoothing passes of the viscosity (10 in every script) */
-    int32_t norm_inf;  /* 0: reference scaled-RMS err; 1: scaled max-norm */
+    int32_t norm_inf;  /* 0: reference scaled-RMS err; 1: scaled max-norm (no reference
+                          counterpart: parity unpinned beyond its definition) */
     int32_t threads;   /* OpenMP threads for the stencil loops (0 = runtime default) */
     int32_t pad_;
     double lx, ly, lz;
