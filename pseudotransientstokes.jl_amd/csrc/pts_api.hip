@@ -61,6 +61,12 @@ struct pts_ctx {
     bool dp_ok = false;
     bool dp_enabled = true;
     StepScalars dp{};
+    // 2D: Musτ is maxloc(μ) with the free-slip copies (checked on the device:
+    // 0 unknown, 1 yes, -1 no), so the kernels may derive it from μ as well.
+    // Any write to μ or Musτ resets the state to unknown.
+    int mx_state = 0;
+    bool mx_enabled = true;
+    int* d_flag = nullptr;
     // field allocations: the pointer handed to kernels may sit `stagger`
     // bytes past the start of its hipMalloc block (see field_alloc)
     std::map<void*, void*> alloc_base;
@@ -72,7 +78,7 @@ struct pts_ctx {
         hipGraphExec_t exec;
         Fields in, out;  // buffers and scalars the graph was captured with
         StepScalars s;
-        bool dp;
+        bool dp, mx;
     };
     std::map<std::pair<int64_t, int>, CachedGraph> graphs;
     int parity = 0;  // flips each single-pass iteration (which physical buffer is `cur`)
@@ -178,6 +184,7 @@ void touched(pts_ctx* c) { c->alt_synced = false; }
 // A write to field f (-1: any field) through the API.
 void wrote(pts_ctx* c, int f) {
     if (f < 0 || f == PTS_MUSTAU || f == PTS_GDTAU || f == PTS_DTAU_RHO) c->dp_ok = false;
+    if (f < 0 || f == PTS_MUSTAU || f == PTS_MUS) c->mx_state = 0;
 }
 
 int setdev(pts_ctx* c) {
@@ -269,12 +276,14 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
     if (const char* e = std::getenv("PTS_DERIVED_PARAMS")) c->dp_enabled = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_DERIVED_MUSTAU")) c->mx_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_STAGGER_BYTES")) c->stagger = std::max<int64_t>(0, std::atoll(e)) / 256 * 256;
     hipError_t e = hipSetDevice(g->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void**)&c->red_partials, sizeof(double) * kRedBlocks * kRedVals);
     if (e == hipSuccess) e = hipMalloc((void**)&c->red_out, sizeof(double) * kRedVals);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_red, sizeof(double) * kRedVals, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&c->d_flag, sizeof(int));
     if (e != hipSuccess) {
         std::string m = std::string("HIP init: ") + hipGetErrorString(e);
         pts_destroy(c);
@@ -304,6 +313,7 @@ int pts_destroy(pts_ctx* c) {
         for (double* p : slot) field_free(c, p);
     if (c->red_partials) (void)hipFree(c->red_partials);
     if (c->red_out) (void)hipFree(c->red_out);
+    if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->h_red) (void)hipHostFree(c->h_red);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -456,6 +466,7 @@ int pts_init_grf_viscosity(pts_ctx* c, int32_t nmodes, const double* modes, doub
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dm);
     if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("grf: ") + hipGetErrorString(e));
+    wrote(c, PTS_MUS);
     return PTS_OK;
 }
 
@@ -691,8 +702,19 @@ double pts_err_from_norms(const pts_norms* n, int32_t ndim, double lx, int32_t k
 // ---- device-resident PT iteration -------------------------------------------
 static bool single_pass(const pts_ctx* c) { return c->nd == 2 || !c->two_kernel_3d; }
 
+static bool use_dp(const pts_ctx* c);
+
 static int prepare_iter(pts_ctx* c) {
     if (int rc = need_iter_fields(c, true)) return rc;
+    if (c->nd == 2 && use_dp(c) && c->mx_enabled && c->mx_state == 0) {
+        if (int rc = ensure(c, PTS_MUSTAU)) return rc;
+        HIPCHK(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+        HIPCHK(c, launch_check_mustau2(c->stream, F(c), c->d_flag));
+        int bad = 1;
+        HIPCHK(c, hipMemcpyAsync(&bad, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->mx_state = bad ? -1 : 1;
+    }
     if (single_pass(c)) {
         int st[16];
         const int ns = state_fields(c, st);
@@ -719,6 +741,7 @@ static void swap_state(pts_ctx* c) {
 // Enqueue n iterations; for the 3D divide form, the last one stores ∇V of
 // the pre-update velocities when store_last_dv is set.
 static bool use_dp(const pts_ctx* c) { return c->dp_ok && c->dp_enabled; }
+static bool use_mx(const pts_ctx* c) { return c->nd == 2 && use_dp(c) && c->mx_enabled && c->mx_state == 1; }
 
 static StepScalars with_dp(const pts_ctx* c, StepScalars s) {
     if (use_dp(c)) {
@@ -751,7 +774,7 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s0, int64_t n, bool stor
         if (single_pass(c)) {
             if (c->nd == 2)
                 HIPCHK(c, launch2_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->rows_per_block,
-                                       c->variant_2d, c->persist_2d, dp));
+                                       c->variant_2d, c->persist_2d, dp, use_mx(c)));
             else
                 HIPCHK(c, launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
                                        c->tile_rows_3d, dp));
@@ -780,7 +803,8 @@ static int run_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_las
     const Fields fin = F(c), fout = fields_of(c, c->alt);
     const StepScalars sd = with_dp(c, s);
     if (it != c->graphs.end() && !(same_fields(it->second.in, fin) && same_fields(it->second.out, fout) &&
-                                   same_scal(it->second.s, sd) && it->second.dp == use_dp(c))) {
+                                   same_scal(it->second.s, sd) && it->second.dp == use_dp(c) &&
+                                   it->second.mx == use_mx(c))) {
         (void)hipGraphExecDestroy(it->second.exec);
         c->graphs.erase(it);
         it = c->graphs.end();
@@ -798,7 +822,7 @@ static int run_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_las
         if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         // capture already swapped the buffers n times: restore, then replay
         for (int64_t q = 0; q < n; ++q) swap_state(c);
-        c->graphs[key] = pts_ctx::CachedGraph{ex, fin, fout, sd, use_dp(c)};
+        c->graphs[key] = pts_ctx::CachedGraph{ex, fin, fout, sd, use_dp(c), use_mx(c)};
         it = c->graphs.find(key);
     }
     HIPCHK(c, hipGraphLaunch(it->second.exec, c->stream));

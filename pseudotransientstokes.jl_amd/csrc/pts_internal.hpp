@@ -162,7 +162,10 @@ hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 // looping over tiles.  variant: 0 plain, 1 prefetch, 2 prefetch + non-temporal
 // stores, 3 non-temporal stores.
 hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
-                        bool mul, bool bf, int rows_per_block, int variant, bool persist, bool dp);
+                        bool mul, bool bf, int rows_per_block, int variant, bool persist, bool dp, bool mx);
+// 1 in *bad unless Musτ equals compute_maxloc! of μ followed by bc_x!, bc_y!
+// (2D; the precondition of the mx iteration kernels).  *bad must be zero.
+hipError_t launch_check_mustau2(hipStream_t s, const Fields& f, int* bad);
 
 // 3D
 hipError_t launch3_compute_P(hipStream_t s, const Fields& f, const StepScalars& c);

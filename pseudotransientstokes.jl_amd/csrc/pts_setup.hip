@@ -137,6 +137,22 @@ __global__ void k_maxloc(double* dst, const double* src, int ndim, int64_t n1, i
     }
 }
 
+// Musτ == maxloc(μ) + free-slip copies, cell by cell (2D, bitwise).
+__global__ void k_check_mustau2(Fields f, int* bad) {
+    const int nx = f.nx, ny = f.ny, sx = f.sx;
+    const int64_t tot = (int64_t)nx * ny;
+    const double* M = f.p[PTS_MUS];
+    const double* Mt = f.p[PTS_MUSTAU];
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % nx), j = (int)(q / nx);
+        const int ic = min(max(i, 1), nx - 2), jc = min(max(j, 1), ny - 2);
+        const int om = ic + sx * jc;
+        const double m = fmax(fmax(fmax(fmax(M[om - 1], M[om + 1]), M[om]), M[om - sx]), M[om + sx]);
+        const double t = Mt[i + sx * j];
+        if (!(m == t)) atomicOr(bad, 1);
+    }
+}
+
 // bc_x!/bc_y!/bc_z!: copy the first interior plane outward along `dim`.
 __global__ void k_bc(double* a, int64_t n1, int64_t n2, int64_t n3, int sx, int sxy, int dim) {
     int64_t e1 = n1, e2 = n2, e3 = n3;
@@ -280,6 +296,12 @@ hipError_t launch_iter_params(hipStream_t s, const Fields& f, int ndim, bool ve,
                               double Re, double r, double ml) {
     const int64_t tot = (int64_t)f.nx * f.ny * (ndim == 3 ? f.nz : 1);
     k_iter_params<<<grid_for(tot), kBlock, 0, s>>>(f, ndim, ve ? 1 : 0, Vp, G, dt, Re, r, ml);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_mustau2(hipStream_t s, const Fields& f, int* bad) {
+    const int64_t tot = (int64_t)f.nx * f.ny;
+    k_check_mustau2<<<grid_for(tot), kBlock, 0, s>>>(f, bad);
     return hipGetLastError();
 }
 

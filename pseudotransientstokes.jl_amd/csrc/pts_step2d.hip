@@ -188,9 +188,20 @@ struct Row2 {
     double Pt, txx, tyy, txo, tyo, Gd, Mu, Rh, VyN, rg, Vx, txy, txyo;
 };
 
-template <bool VE, bool BF, bool DP>
+// Musτ of cell (cx, J) from μ: compute_maxloc! (the 5-point cross maximum,
+// Stokes2D_ve.jl:23-26, in the order of k_maxloc) at the cell clamped into
+// the interior, which is what bc_x!, bc_y! leave on the boundary cells
+// (Stokes2D_ve.jl:142-145).
+__device__ __forceinline__ double mustau_of(const Fields& in, int cx, int J) {
+    const int cc = min(max(cx, 1), in.nx - 2), jj = min(max(J, 1), in.ny - 2);
+    const int om = cc + in.sx * jj;
+    const double* M = in.p[PTS_MUS];
+    return fmax(fmax(fmax(fmax(M[om - 1], M[om + 1]), M[om]), M[om - in.sx]), M[om + in.sx]);
+}
+
+template <bool VE, bool BF, bool DP, bool MX>
 __device__ __forceinline__ void load_row2(Row2& w, const Fields& in, const StepScalars& c, int o, int sx, bool cin,
-                                          bool fin, bool vtx) {
+                                          bool fin, bool vtx, int cx, int J) {
     w = Row2{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (cin) {
         w.Pt = in.p[PTS_PT][o];
@@ -201,7 +212,7 @@ __device__ __forceinline__ void load_row2(Row2& w, const Fields& in, const StepS
             w.tyo = in.p[PTS_TYY_O][o];
         }
         if constexpr (DP) {
-            iter_params_of(in.p[PTS_MUSTAU][o], c, w.Rh, w.Gd);
+            iter_params_of(MX ? mustau_of(in, cx, J) : in.p[PTS_MUSTAU][o], c, w.Rh, w.Gd);
         } else {
             w.Gd = in.p[PTS_GDTAU][o];
             w.Rh = in.p[PTS_DTAU_RHO][o];
@@ -222,7 +233,7 @@ __device__ __forceinline__ void load_row2(Row2& w, const Fields& in, const StepS
 // exactly the row it needs.  Lanes outside the grid then hold copies of edge
 // values; no stored output depends on them (each output's operands come from
 // lanes and rows inside its own validity range).
-template <bool VE, bool BF, bool DP>
+template <bool VE, bool BF, bool DP, bool MX>
 __device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, const StepScalars& c, int cx, int J,
                                                   int nx, int sx) {
     const int oc = min(max(cx, 0), nx - 1) + sx * J;
@@ -238,7 +249,7 @@ __device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, con
         w.txyo = in.p[PTS_TXY_O][ov];
     }
     if constexpr (DP) {
-        iter_params_of(in.p[PTS_MUSTAU][oc], c, w.Rh, w.Gd);
+        iter_params_of(MX ? mustau_of(in, cx, J) : in.p[PTS_MUSTAU][oc], c, w.Rh, w.Gd);
     } else {
         w.Gd = in.p[PTS_GDTAU][oc];
         w.Rh = in.p[PTS_DTAU_RHO][oc];
@@ -276,8 +287,8 @@ __device__ __forceinline__ void st(double* p, double v) {
 // first and last lane are halo), so a row of each array is read as one
 // contiguous run of 64·NW values; values crossing the wave boundaries inside
 // the block go through LDS (two barriers per row step).
-template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP>
-__global__ __launch_bounds__(64 * NW) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, bool MX>
+__global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
                                                    int nstrips, int ntiles) {
     constexpr int W = 64 * NW - 2;  // output columns per strip
     constexpr int NXL = 6;          // values passed to the right neighbour wave
@@ -434,17 +445,17 @@ __global__ __launch_bounds__(64 * NW) void k2_iter(const Fields in, const Fields
         };
         Row2 ra, rb;
         if constexpr (PF) {
-            load_row2_clamped<VE, BF, DP>(ra, in, c, cx, j0, nx, sx);
+            load_row2_clamped<VE, BF, DP, MX>(ra, in, c, cx, j0, nx, sx);
             for (int J = j0; J <= j1; J += 2) {
-                if (J + 1 <= j1) load_row2_clamped<VE, BF, DP>(rb, in, c, cx, J + 1, nx, sx);
+                if (J + 1 <= j1) load_row2_clamped<VE, BF, DP, MX>(rb, in, c, cx, J + 1, nx, sx);
                 step(J, ra);
                 if (J + 1 > j1) break;
-                if (J + 2 <= j1) load_row2_clamped<VE, BF, DP>(ra, in, c, cx, J + 2, nx, sx);
+                if (J + 2 <= j1) load_row2_clamped<VE, BF, DP, MX>(ra, in, c, cx, J + 2, nx, sx);
                 step(J + 1, rb);
             }
         } else {
             for (int J = j0; J <= j1; ++J) {
-                load_row2<VE, BF, DP>(ra, in, c, cx + sx * J, sx, cin, fin, vin && J >= 1);
+                load_row2<VE, BF, DP, MX>(ra, in, c, cx + sx * J, sx, cin, fin, vin && J >= 1, cx, J);
                 step(J, ra);
             }
         }
@@ -486,24 +497,24 @@ hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 }
 
 // Resident blocks of one k2_iter instance on the whole device.
-template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP>
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, bool MX>
 static int concurrency2() {
     static int cached = 0;
     if (!cached) {
         int dev = 0, cus = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k2_iter<VE, MUL, BF, PF, NT, NW, DP>, 64 * NW, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k2_iter<VE, MUL, BF, PF, NT, NW, DP, MX>, 64 * NW, 0);
         cached = std::max(1, cus) * std::max(1, per);
     }
     return cached;
 }
 
-template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW = 1, bool DP = false>
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW = 1, bool DP = false, bool MX = false>
 static void go2v(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, bool persist) {
     constexpr int W = 64 * NW - 2;
     const int ns = (in.nx + 1 + W - 1) / W;
-    const int conc = concurrency2<VE, MUL, BF, PF, NT, NW, DP>();
+    const int conc = concurrency2<VE, MUL, BF, PF, NT, NW, DP, MX>();
     if (L < 1) {  // about one round of resident waves (L < 0: within [4, -L] rows)
         const int nch = std::max(1, conc / ns);
         const int cap = -L;
@@ -512,14 +523,17 @@ static void go2v(hipStream_t s, const Fields& in, const Fields& out, const StepS
     }
     const int ntiles = ns * ((in.ny + L - 1) / L);
     const int grid = persist ? std::min(ntiles, conc) : ntiles;
-    k2_iter<VE, MUL, BF, PF, NT, NW, DP><<<grid, 64 * NW, 0, s>>>(in, out, c, L, ns, ntiles);
+    k2_iter<VE, MUL, BF, PF, NT, NW, DP, MX><<<grid, 64 * NW, 0, s>>>(in, out, c, L, ns, ntiles);
 }
 
-// dp: derive Gdτ, dτ_Rho from Musτ (variants 1 and 7; the others read them)
+// dp: derive Gdτ, dτ_Rho from Musτ (variants 1, 7 and 10; the others read
+// them); mx: derive Musτ from μ as well (variant 7)
 template <bool VE, bool MUL, bool BF>
 static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, int variant,
-                bool persist, bool dp) {
+                bool persist, bool dp, bool mx) {
+    if (dp && mx && variant == 7) return go2v<VE, MUL, BF, true, false, 8, true, true>(s, in, out, c, L, persist);
     if (dp && variant == 7) return go2v<VE, MUL, BF, true, false, 8, true>(s, in, out, c, L, persist);
+    if (dp && variant == 10) return go2v<VE, MUL, BF, false, false, 8, true>(s, in, out, c, L, persist);
     if (dp && variant == 1) return go2v<VE, MUL, BF, true, false, 1, true>(s, in, out, c, L, persist);
     switch (variant) {
         case 0: go2v<VE, MUL, BF, false, false>(s, in, out, c, L, persist); break;
@@ -536,13 +550,13 @@ static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepSc
 }
 
 hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
-                        bool mul, bool bf, int L, int variant, bool persist, bool dp) {
+                        bool mul, bool bf, int L, int variant, bool persist, bool dp, bool mx) {
     if (ve) {
-        if (mul) { if (bf) go2<true, true, true>(s, in, out, c, L, variant, persist, dp); else go2<true, true, false>(s, in, out, c, L, variant, persist, dp); }
-        else { if (bf) go2<true, false, true>(s, in, out, c, L, variant, persist, dp); else go2<true, false, false>(s, in, out, c, L, variant, persist, dp); }
+        if (mul) { if (bf) go2<true, true, true>(s, in, out, c, L, variant, persist, dp, mx); else go2<true, true, false>(s, in, out, c, L, variant, persist, dp, mx); }
+        else { if (bf) go2<true, false, true>(s, in, out, c, L, variant, persist, dp, mx); else go2<true, false, false>(s, in, out, c, L, variant, persist, dp, mx); }
     } else {
-        if (mul) { if (bf) go2<false, true, true>(s, in, out, c, L, variant, persist, dp); else go2<false, true, false>(s, in, out, c, L, variant, persist, dp); }
-        else { if (bf) go2<false, false, true>(s, in, out, c, L, variant, persist, dp); else go2<false, false, false>(s, in, out, c, L, variant, persist, dp); }
+        if (mul) { if (bf) go2<false, true, true>(s, in, out, c, L, variant, persist, dp, mx); else go2<false, true, false>(s, in, out, c, L, variant, persist, dp, mx); }
+        else { if (bf) go2<false, false, true>(s, in, out, c, L, variant, persist, dp, mx); else go2<false, false, false>(s, in, out, c, L, variant, persist, dp, mx); }
     }
     return hipGetLastError();
 }

@@ -88,6 +88,13 @@ def test_uploaded_iteration_parameters_are_read(preset, kw):
     o.iterate(2)
     assert compare(m, o, STATE) == []
     rng = np.random.default_rng(4)
+    # μ changed without recomputing Musτ: Musτ is no longer maxloc(μ)
+    mu = o.get("Mus") * rng.uniform(0.5, 2.0, size=o.shape("Mus"))
+    o.set("Mus", mu)
+    m.ctx.upload("Mus", mu)
+    m.iterate(3)
+    o.iterate(3)
+    assert compare(m, o, STATE) == []
     for f, lo, hi in (("Gdtau", 0.05, 0.5), ("dtau_Rho", 0.001, 0.02)):
         a = rng.uniform(lo, hi, size=o.shape(f))
         o.set(f, a)
@@ -152,6 +159,9 @@ VARIANTS = [
     ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_2D_VARIANT": "1"}),
     ("Stokes2D", dict(nx=1100, ny=45, rhog_in=-1.0), {"PTS_2D_VARIANT": "7"}),
     ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_DERIVED_PARAMS": "0"}),
+    ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_DERIVED_MUSTAU": "0"}),
+    ("Stokes2D_ve_perf", dict(nx=1021, ny=37), {"PTS_2D_VARIANT": "10"}),
+    ("Stokes2D", dict(nx=511, ny=20, rhog_in=-1.0), {"PTS_ROWS_PER_BLOCK": "-4"}),
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_DERIVED_PARAMS": "0"}),
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_STAGGER_BYTES": "0"}),
     ("Stokes3D_ve_multixpu", dict(n=17), {"PTS_STAGGER_BYTES": "65792"}),
