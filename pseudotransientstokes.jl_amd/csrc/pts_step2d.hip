@@ -199,7 +199,7 @@ __device__ __forceinline__ double mustau_of(const Fields& in, int cx, int J) {
     return fmax(fmax(fmax(fmax(M[om - 1], M[om + 1]), M[om]), M[om - in.sx]), M[om + in.sx]);
 }
 
-template <bool VE, bool BF, bool DP, bool MX>
+template <bool VE, bool BF, bool DP, int MX>
 __device__ __forceinline__ void load_row2(Row2& w, const Fields& in, const StepScalars& c, int o, int sx, bool cin,
                                           bool fin, bool vtx, int cx, int J) {
     w = Row2{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -233,7 +233,7 @@ __device__ __forceinline__ void load_row2(Row2& w, const Fields& in, const StepS
 // exactly the row it needs.  Lanes outside the grid then hold copies of edge
 // values; no stored output depends on them (each output's operands come from
 // lanes and rows inside its own validity range).
-template <bool VE, bool BF, bool DP, bool MX>
+template <bool VE, bool BF, bool DP, int MX>
 __device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, const StepScalars& c, int cx, int J,
                                                   int nx, int sx) {
     const int oc = min(max(cx, 0), nx - 1) + sx * J;
@@ -248,7 +248,9 @@ __device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, con
         w.tyo = in.p[PTS_TYY_O][oc];
         w.txyo = in.p[PTS_TXY_O][ov];
     }
-    if constexpr (DP) {
+    if constexpr (DP && MX == 2) {
+        w.Gd = w.Rh = 0.0;  // derived from μ in the row step
+    } else if constexpr (DP) {
         iter_params_of(MX ? mustau_of(in, cx, J) : in.p[PTS_MUSTAU][oc], c, w.Rh, w.Gd);
     } else {
         w.Gd = in.p[PTS_GDTAU][oc];
@@ -287,12 +289,19 @@ __device__ __forceinline__ void st(double* p, double v) {
 // first and last lane are halo), so a row of each array is read as one
 // contiguous run of 64·NW values; values crossing the wave boundaries inside
 // the block go through LDS (two barriers per row step).
-template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, bool MX>
+//
+// MX: Musτ derived from μ (Gdτ, dτ_Rho from Musτ need DP).  MX = 1 loads the
+// five μ of each cell's cross; MX = 2 (with PF) takes them from the row
+// pipeline: x-neighbours by DPP (wave edges: exchanged one row ahead through
+// LDS), the row below from registers, the row above from the prefetched row;
+// rows whose cross is not in the pipeline (first and last row of a chunk,
+// grid row 0) and the boundary columns fall back to the loads.
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, int MX>
 __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
                                                    int nstrips, int ntiles) {
     constexpr int W = 64 * NW - 2;  // output columns per strip
-    constexpr int NXL = 6;          // values passed to the right neighbour wave
-    __shared__ double sL[NW > 1 ? NW : 1][NXL], sR[NW > 1 ? NW : 1], sT[NW > 1 ? NW : 1];
+    constexpr int NXL = 7;          // values passed to the right neighbour wave
+    __shared__ double sL[NW > 1 ? NW : 1][NXL], sR[NW > 1 ? NW : 1][2], sT[NW > 1 ? NW : 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, gl = threadIdx.x;
     const int nx = in.nx, ny = in.ny, sx = in.sx;
     double* __restrict__ Pt1 = out.p[PTS_PT];
@@ -323,12 +332,29 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
         // values carried from row J-1
         double VxP = 0.0, GdP = 0.0, MuP = 0.0, GdPl = 0.0, MuPl = 0.0, RhP = 0.0, rgP = 0.0;
         double Pt1P = 0.0, txx1P = 0.0, tyy1P = 0.0, txy1PP = 0.0;
+        double MtP = 0.0, muLn = 0.0, muRn = 0.0;  // MX == 2: Musτ of row J-1; edge μ of row J
         const int j0 = max(jb - 1, 0), j1 = min(je, ny - 1);
         double VyC = in.p[PTS_VY][min(max(cx, 0), nx - 1) + sx * j0];  // Vy0[c, J]
 
-        auto step = [&](const int J, const Row2& cur) {
+        auto step = [&](const int J, Row2 cur, const double nxtMu) {
             const int o = cx + sx * J;
             const bool vtx = vin && J >= 1;
+            if constexpr (DP && MX == 2) {
+                double muL = from_left(cur.Mu), muR = from_right(cur.Mu);
+                if (NW > 1 && lane == 0 && wv > 0) muL = muLn;
+                if (NW > 1 && lane == 63 && wv < NW - 1) muR = muRn;
+                double Mt;
+                if (J == 0 || J == j0 || (J == j1 && J < ny - 1)) {
+                    Mt = mustau_of(in, cx, J);
+                } else if (J == ny - 1) {
+                    Mt = MtP;  // Musτ(:, ny-1) = Musτ(:, ny-2)
+                } else {
+                    Mt = fmax(fmax(fmax(fmax(muL, muR), cur.Mu), MuP), nxtMu);
+                    if (cx == 0 || cx == nx - 1) Mt = mustau_of(in, cx, J);  // clamped column
+                }
+                MtP = Mt;
+                iter_params_of(Mt, c, cur.Rh, cur.Gd);
+            }
             // lane exchanges (whole wave active)
             double VxR = from_right(cur.Vx);
             double GdL = from_left(cur.Gd);
@@ -345,8 +371,12 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
                     sL[wv][3] = txx1P;
                     sL[wv][4] = Pt1P;
                     sL[wv][5] = RhP;
+                    sL[wv][6] = nxtMu;
                 }
-                if (lane == 0) sR[wv] = cur.Vx;
+                if (lane == 0) {
+                    sR[wv][0] = cur.Vx;
+                    sR[wv][1] = nxtMu;
+                }
                 __syncthreads();
                 if (lane == 0 && wv > 0) {
                     GdL = sL[wv - 1][0];
@@ -355,8 +385,12 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
                     txx1PL = sL[wv - 1][3];
                     Pt1PL = sL[wv - 1][4];
                     RhPL = sL[wv - 1][5];
+                    muLn = sL[wv - 1][6];
                 }
-                if (lane == 63 && wv < NW - 1) VxR = sR[wv + 1];
+                if (lane == 63 && wv < NW - 1) {
+                    VxR = sR[wv + 1][0];
+                    muRn = sR[wv + 1][1];
+                }
             }
 
             // cell (c, J)
@@ -448,15 +482,15 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
             load_row2_clamped<VE, BF, DP, MX>(ra, in, c, cx, j0, nx, sx);
             for (int J = j0; J <= j1; J += 2) {
                 if (J + 1 <= j1) load_row2_clamped<VE, BF, DP, MX>(rb, in, c, cx, J + 1, nx, sx);
-                step(J, ra);
+                step(J, ra, J + 1 <= j1 ? rb.Mu : ra.Mu);
                 if (J + 1 > j1) break;
                 if (J + 2 <= j1) load_row2_clamped<VE, BF, DP, MX>(ra, in, c, cx, J + 2, nx, sx);
-                step(J + 1, rb);
+                step(J + 1, rb, J + 2 <= j1 ? ra.Mu : rb.Mu);
             }
         } else {
             for (int J = j0; J <= j1; ++J) {
                 load_row2<VE, BF, DP, MX>(ra, in, c, cx + sx * J, sx, cin, fin, vin && J >= 1, cx, J);
-                step(J, ra);
+                step(J, ra, ra.Mu);
             }
         }
     }
@@ -497,7 +531,7 @@ hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 }
 
 // Resident blocks of one k2_iter instance on the whole device.
-template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, bool MX>
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, int MX>
 static int concurrency2() {
     static int cached = 0;
     if (!cached) {
@@ -510,7 +544,7 @@ static int concurrency2() {
     return cached;
 }
 
-template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW = 1, bool DP = false, bool MX = false>
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW = 1, bool DP = false, int MX = 0>
 static void go2v(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, bool persist) {
     constexpr int W = 64 * NW - 2;
     const int ns = (in.nx + 1 + W - 1) / W;
@@ -531,7 +565,8 @@ static void go2v(hipStream_t s, const Fields& in, const Fields& out, const StepS
 template <bool VE, bool MUL, bool BF>
 static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int L, int variant,
                 bool persist, bool dp, bool mx) {
-    if (dp && mx && variant == 7) return go2v<VE, MUL, BF, true, false, 8, true, true>(s, in, out, c, L, persist);
+    if (dp && mx && variant == 7) return go2v<VE, MUL, BF, true, false, 8, true, 2>(s, in, out, c, L, persist);
+    if (dp && mx && variant == 11) return go2v<VE, MUL, BF, true, false, 8, true, 1>(s, in, out, c, L, persist);
     if (dp && variant == 7) return go2v<VE, MUL, BF, true, false, 8, true>(s, in, out, c, L, persist);
     if (dp && variant == 10) return go2v<VE, MUL, BF, false, false, 8, true>(s, in, out, c, L, persist);
     if (dp && variant == 1) return go2v<VE, MUL, BF, true, false, 1, true>(s, in, out, c, L, persist);

@@ -160,6 +160,10 @@ VARIANTS = [
     ("Stokes2D", dict(nx=1100, ny=45, rhog_in=-1.0), {"PTS_2D_VARIANT": "7"}),
     ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_DERIVED_PARAMS": "0"}),
     ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_DERIVED_MUSTAU": "0"}),
+    ("Stokes2D_ve_perf", dict(nx=1100, ny=75), {"PTS_2D_VARIANT": "11"}),
+    ("Stokes2D_ve_perf", dict(nx=1019, ny=100), {"PTS_ROWS_PER_BLOCK": "7"}),
+    ("Stokes2D", dict(nx=509, ny=66, rhog_in=-1.0), {"PTS_ROWS_PER_BLOCK": "5"}),
+    ("Stokes2D_ve", dict(nx=1021, ny=9), {"PTS_ROWS_PER_BLOCK": "2"}),
     ("Stokes2D_ve_perf", dict(nx=1021, ny=37), {"PTS_2D_VARIANT": "10"}),
     ("Stokes2D", dict(nx=511, ny=20, rhog_in=-1.0), {"PTS_ROWS_PER_BLOCK": "-4"}),
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_DERIVED_PARAMS": "0"}),
