@@ -203,7 +203,8 @@ def bench_c5(n=2047, steps=200):
                 forward_ms_per_it=round(f_it, 4), forward_T_eff_GBs=round(a_eff / f_it / 1e6, 1),
                 adjoint_ms_per_it=round(a_it, 4), adjoint_T_eff_GBs=round(a_eff / a_it / 1e6, 1),
                 A_eff_bytes=a_eff, note="T_eff with A_eff = 15 arrays (6 state read+written, 3 read-only); the "
-                                       "adjoint moves 29 arrays per iteration in two passes")
+                                       "single-pass adjoint sweep reads the 6 adjoint fields, mu, Gd, dtau_Rho and "
+                                       "the dJ/dVx source, and writes 6")
 
 
 def cpu_baseline(n, iters, threads):

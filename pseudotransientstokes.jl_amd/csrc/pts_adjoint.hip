@@ -290,12 +290,173 @@ __global__ __launch_bounds__(256) void k_max2_final(const double* part, int nb, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// One adjoint iteration in a single pass (k2_adj1 and k2_adj2 fused): one
+// 64-lane wave per strip of 60 columns (lanes 0, 1, 62, 63 are halo: a
+// velocity entry of λ' needs pass-1 values two columns to its left), marching
+// a chunk of rows in y after two warm-up rows.  At row step J lane c forms
+//   wx at face (c, J) and wy at face (c, J+1)   (pass 1, folded copies)
+//   l1 of cell (c, J) and of vertex (c, J-1)    (pass 1)
+//   λ' of cell (c, J), vertex (c, J-1), Vx face (c, J-1), Vy face (c, J)
+// with the expressions of k2_adj1 / k2_adj2 operation for operation, so both
+// forms give the same bits.  Values of the previous row are carried in
+// registers, x-neighbours come from DPP lane shifts.
+// ---------------------------------------------------------------------------
+constexpr int kAdjW = 60;
+
+__global__ __launch_bounds__(64) void k2_adj_iter(Fields f, Fields lam, Fields out, StepScalars cs, double igdt,
+                                                  Fields g, int L, int nstrips) {
+    const int lane = threadIdx.x;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int strip = b % nstrips, chunk = b / nstrips;
+    const int c = strip * kAdjW - 2 + lane;
+    const int nx = f.nx, ny = f.ny, sx = f.sx;
+    const int jb = chunk * L, je = min(jb + L, ny);
+    const bool own = lane >= 2 && lane <= kAdjW + 1;
+    const int cc = min(max(c, 0), nx - 1), cf = min(max(c, 0), nx), cv = min(max(c, 0), nx - 2);
+    const double _dx = cs._dx, _dy = cs._dy, r = cs.r;
+    const double* __restrict__ LP = lam.p[PTS_PT];
+    const double* __restrict__ Lxx = lam.p[PTS_TXX];
+    const double* __restrict__ Lyy = lam.p[PTS_TYY];
+    const double* __restrict__ Lxy = lam.p[PTS_TXY];
+    const double* __restrict__ LVx = lam.p[PTS_VX];
+    const double* __restrict__ LVy = lam.p[PTS_VY];
+    const double* __restrict__ Gdp = f.p[PTS_GDTAU];
+    const double* __restrict__ Mup = f.p[PTS_MUS];
+    const double* __restrict__ Rhp = f.p[PTS_DTAU_RHO];
+    const double *gx = g.p[PTS_VX], *gy = g.p[PTS_VY], *gxx = g.p[PTS_TXX], *gyy = g.p[PTS_TYY],
+                 *gxy = g.p[PTS_TXY];
+    const bool cx_in = c >= 1 && c <= nx - 1, cy_in = c >= 1 && c <= nx - 2;
+    const bool cell_c = c >= 0 && c < nx, vert_c = c >= 0 && c <= nx - 2;
+
+    // carried from row J-1 (face rows of wy/fy: row J)
+    double wyC = 0.0, fyC = 0.0, wxP = 0.0, fxP = 0.0, lVxP = 0.0;
+    double qP = 0.0, pxxP = 0.0, pyyP = 0.0, pxyP = 0.0, GdP = 0.0, MuP = 0.0;
+    const int J0 = max(jb - 2, 0), J1 = min(je, ny);
+    double RhC = Rhp[cc + sx * min(J0, ny - 1)];
+    for (int J = J0; J <= J1; ++J) {
+        const int jc = min(J, ny - 1);
+        const int oc = cc + sx * jc;
+        const double lP = LP[oc], lxx = Lxx[oc], lyy = Lyy[oc], Gd = Gdp[oc], Mu = Mup[oc];
+        const double lVx = LVx[cf + sx * jc];
+        const double lVyN = LVy[cc + sx * min(J + 1, ny)];
+        const double RhN = Rhp[cc + sx * min(J + 1, ny - 1)];
+        const double lxy = Lxy[cv + sx * min(max(J - 1, 0), ny - 2)];
+        const double lVxNx = (J == ny - 2) ? LVx[cf + sx * (ny - 1)] : 0.0;
+
+        // pass 1: weights at face (c, J) of Vx and face (c, J+1) of Vy
+        const bool xin = cx_in && J >= 1 && J <= ny - 2;
+        double fx = 0.0;
+        if (xin) {
+            fx = lVx;
+            if (J == 1) fx += lVxP;
+            if (J == ny - 2) fx += lVxNx;
+        }
+        const double RhL = from_left(RhC);
+        const double wx = xin ? fx * (0.5 * (RhL + RhC)) : 0.0;
+        const bool yin = cy_in && J + 1 <= ny - 1;
+        const double lVyNL = from_left(lVyN), lVyNR = from_right(lVyN);
+        double fyN = 0.0;
+        if (yin) {
+            fyN = lVyN;
+            if (c == 1) fyN += lVyNL;
+            if (c == nx - 2) fyN += lVyNR;
+        }
+        const double wyN = yin ? fyN * (0.5 * (RhC + RhN)) : 0.0;
+        // pass 1: cell (c, J) and vertex (c, J-1)
+        const double wxR = from_right(wx), wxPR = from_right(wxP), wyCR = from_right(wyC);
+        const double l1P = lP + _dx * (wxR - wx) + _dy * (wyN - wyC);
+        const double l1xx = lxx + _dx * (wx - wxR);
+        const double l1yy = lyy + _dy * (wyC - wyN);
+        const double l1xy = lxy + _dy * (wxPR - wxR) + _dx * (wyC - wyCR);
+        // pass 2 coefficients: cell (c, J) and vertex (c, J-1)
+        const double a = 1.0 / (1.0 + Gd / Mu + Gd * igdt);
+        const double GdR = from_right(Gd), MuR = from_right(Mu), GdPR = from_right(GdP), MuPR = from_right(MuP);
+        const double ag = 0.25 * (GdP + GdPR + Gd + GdR);
+        const double am = 0.25 * (MuP + MuPR + Mu + MuR);
+        const double av = 1.0 / (1.0 + ag / am + ag * igdt);
+        const double bv = 2.0 * ag * av;
+        const bool cell = cell_c && J < ny;
+        const bool vert = vert_c && J >= 1 && J - 1 <= ny - 2;
+        const double q = cell ? r * Gd * l1P : 0.0;
+        const double pxx = cell ? 2.0 * Gd * a * l1xx : 0.0;
+        const double pyy = cell ? 2.0 * Gd * a * l1yy : 0.0;
+        const double pxy = vert ? bv * l1xy : 0.0;
+        const double qPL = from_left(qP), pxxPL = from_left(pxxP), pxyPL = from_left(pxyP), pxyL = from_left(pxy);
+
+        if (own) {
+            if (cell && J >= jb && J < je) {
+                const int o = c + sx * J;
+                double txx = a * l1xx, tyy = a * l1yy;
+                if (gxx) txx += gxx[o];
+                if (gyy) tyy += gyy[o];
+                out.p[PTS_PT][o] = l1P;
+                out.p[PTS_TXX][o] = txx;
+                out.p[PTS_TYY][o] = tyy;
+            }
+            if (vert && J - 1 >= jb && J - 1 < je) {
+                const int o = c + sx * (J - 1);
+                double txy = av * l1xy;
+                if (gxy) txy += gxy[o];
+                out.p[PTS_TXY][o] = txy;
+            }
+            if (J >= 1 && J - 1 >= jb && J - 1 < je && c >= 0 && c <= nx) {  // Vx face (c, J-1)
+                const int o = c + sx * (J - 1);
+                double v = 0.0;
+                if (c >= 1 && c <= nx - 1) {
+                    v = fxP - _dx * (qPL - qP) + _dx * (pxxPL - pxxP) + 0.5 * _dy * (pxyPL - pxyL);
+                    if (gx) v += gx[o];
+                }
+                out.p[PTS_VX][o] = v;
+            }
+            if (((J >= jb && J < je) || (J == ny && je == ny)) && c >= 0 && c < nx) {  // Vy face (c, J)
+                const int o = c + sx * J;
+                double v = 0.0;
+                if (J >= 1 && J <= ny - 1) {
+                    v = fyC - _dy * (qP - q) + _dy * (pyyP - pyy) + 0.5 * _dx * (pxyL - pxy);
+                    if (gy) v += gy[o];
+                }
+                out.p[PTS_VY][o] = v;
+            }
+        }
+        // carry
+        lVxP = lVx;
+        RhC = RhN;
+        wyC = wyN;
+        fyC = fyN;
+        wxP = wx;
+        fxP = fx;
+        qP = q;
+        pxxP = pxx;
+        pyyP = pyy;
+        pxyP = pxy;
+        GdP = Gd;
+        MuP = Mu;
+    }
+}
+
 inline dim3 grid2(int n1, int n2) { return dim3((n1 + BX - 1) / BX, (n2 + BY - 1) / BY, 1); }
 
 }  // namespace
 
 hipError_t launch2_adj_step(hipStream_t s, const Fields& f, const Fields& lam, const Fields& tmp, const Fields& out,
-                            const StepScalars& c, double igdt, const Fields& g) {
+                            const StepScalars& c, double igdt, const Fields& g, bool fused) {
+    if (fused) {
+        static int conc = 0;
+        if (!conc) {
+            int dev = 0, cus = 0, per = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k2_adj_iter, 64, 0);
+            conc = std::max(1, cus) * std::max(1, per);
+        }
+        const int ns = (f.nx + 1 + kAdjW - 1) / kAdjW;
+        const int nch = std::max(1, conc / ns);
+        const int L = std::max(8, std::min(64, (f.ny + nch - 1) / nch));
+        const int nb = ns * ((f.ny + L - 1) / L);
+        k2_adj_iter<<<nb, 64, 0, s>>>(f, lam, out, c, igdt, g, L, ns);
+        return hipGetLastError();
+    }
     k2_adj1<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, lam, tmp, c);
     k2_adj2<<<grid2(f.nx + 1, f.ny + 1), dim3(BX, BY), 0, s>>>(f, tmp, out, c, igdt, g);
     return hipGetLastError();

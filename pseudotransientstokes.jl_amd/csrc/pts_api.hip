@@ -54,6 +54,7 @@ struct pts_ctx {
     double* adj_src[PTS_NFIELDS] = {};  // ∂J/∂x under VX, VY; step coupling under TXX, TYY, TXY
     double* adj_grad = nullptr;
     int adj_cur = 0;
+    bool adj_fused = true;  // single-pass adjoint sweep (PTS_ADJ_TWO_PASS=1: the two passes)
     std::vector<std::array<double*, 3>> ckpt;  // converged τxx, τyy, τxy of each physical step
     // Gdτ and dτ_Rho as produced by the last pts_iter_params from the current
     // Musτ: the single-pass kernels may then derive them from Musτ (one array
@@ -277,6 +278,7 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
     if (const char* e = std::getenv("PTS_DERIVED_PARAMS")) c->dp_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_DERIVED_MUSTAU")) c->mx_enabled = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_ADJ_TWO_PASS")) c->adj_fused = std::atoi(e) == 0;
     if (const char* e = std::getenv("PTS_STAGGER_BYTES")) c->stagger = std::max<int64_t>(0, std::atoll(e)) / 256 * 256;
     hipError_t e = hipSetDevice(g->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1116,7 +1118,7 @@ static int adj_steps(pts_ctx* c, const StepScalars& s, double igdt, const Fields
     for (int64_t q = 0; q < n; ++q) {
         const int a = c->adj_cur;
         HIPCHK(c, launch2_adj_step(c->stream, f, adj_fields(c, c->adj[a]), t, adj_fields(c, c->adj[a ^ 1]), s, igdt,
-                                   g));
+                                   g, c->adj_fused));
         c->adj_cur = a ^ 1;
     }
     return PTS_OK;

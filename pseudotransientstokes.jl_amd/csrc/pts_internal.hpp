@@ -195,8 +195,9 @@ hipError_t launch_dc_norms(hipStream_t s, const Fields32& g, const StepScalars32
 // the adjoint fields under the forward field ids (PT, TXX, TYY, TXY, VX, VY).
 // igdt = 1/(G·dt) for the visco-elastic τ update, 0 for viscous; g carries
 // the sources under the field ids VX, VY, TXX, TYY, TXY (null pointers: none).
+// fused: the single-pass sweep (same bits as the two passes; tmp unused)
 hipError_t launch2_adj_step(hipStream_t s, const Fields& f, const Fields& lam, const Fields& tmp, const Fields& out,
-                            const StepScalars& c, double igdt, const Fields& g);
+                            const StepScalars& c, double igdt, const Fields& g, bool fused);
 hipError_t launch2_adj_pass1(hipStream_t s, const Fields& f, const Fields& lam, const Fields& tmp,
                              const StepScalars& c);
 hipError_t launch2_adj_grad(hipStream_t s, const Fields& f, const Fields& l1, double igdt, double* grad, bool acc);

@@ -98,6 +98,30 @@ def test_adjoint_step_is_transpose_of_oracle_iteration(nx, ny, ve):
     c.close()
 
 
+@pytest.mark.parametrize("nx,ny,ve", [(131, 70, 0), (61, 200, 1), (7, 5, 0), (300, 9, 1)])
+def test_fused_adjoint_sweep_equals_two_passes(nx, ny, ve, monkeypatch):
+    """The single-pass adjoint sweep gives the bits of the two-pass form,
+    with V sources (and, visco-elastic, over a few iterations)."""
+    rng = np.random.default_rng(nx + ny)
+    coef = {"Mus": np.exp(rng.normal(0.0, 1.0, (nx, ny))), "Gdtau": rng.uniform(0.2, 2.0, (nx, ny)),
+            "dtau_Rho": rng.uniform(0.01, 0.2, (nx, ny))}
+    out = []
+    for two in ("0", "1"):
+        monkeypatch.setenv("PTS_ADJ_TWO_PASS", two)
+        c = Context(2, nx, ny, ve=bool(ve), fd_mode="mul")
+        for n, a in coef.items():
+            c.upload(n, a)
+        r2 = np.random.default_rng(1)
+        c.adjoint_set_source(r2.normal(size=(nx + 1, ny)), r2.normal(size=(nx, ny + 1)))
+        for n in ADJ:
+            c.adjoint_upload(n, r2.normal(size=c.shape(n)) * state_mask(n, c.shape(n)))
+        c.adjoint_iterate(Context.pt_params(0.9, 0.7, 0.5, 10.0 / nx, 10.0 / ny), 3)
+        out.append({n: c.adjoint_download(n) for n in ADJ})
+        c.close()
+    for n in ADJ:
+        assert np.array_equal(out[0][n], out[1][n]), n
+
+
 def test_adjoint_source_enters_every_step():
     """λ = Φᵀλ + g: with λ = 0, one step returns g on the state entries."""
     nx, ny = 12, 9
