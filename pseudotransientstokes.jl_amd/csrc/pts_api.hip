@@ -9,6 +9,7 @@
 #include <cstring>
 #include <array>
 #include <map>
+#include <set>
 #include <vector>
 #include <string>
 
@@ -44,9 +45,11 @@ struct pts_ctx {
     int rows_per_block = -32;   // 2D single-pass: rows per block (0: one round of resident blocks; <0: that, capped)
     int variant_2d = 7;         // 2D single-pass code variant (prefetch, 8-wave strips)
     bool persist_2d = false;    // 2D single-pass: persistent waves looping over tiles
-    int planes_per_block = -24; // 3D single-pass: z planes per block (<0: one round of resident blocks, capped)
+    int planes_per_block = -160; // 3D single-pass: z planes per block (<0: fewest block rounds, capped)
     int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
+    int variant_3d = 2;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
+    bool group_3d = true;       // 3D: state fields, partners and τ_o in grouped allocations (ensure_group)
     // adjoint state λ (ping-pong by field id), the pass-1 scratch, the
     // source ∂J/∂Vx, ∂J/∂Vy and the gradient buffer
     double* adj[2][PTS_NFIELDS] = {};
@@ -71,6 +74,8 @@ struct pts_ctx {
     // field allocations: the pointer handed to kernels may sit `stagger`
     // bytes past the start of its hipMalloc block (see field_alloc)
     std::map<void*, void*> alloc_base;
+    std::set<void*> slot_ptrs;  // slots 1, 2 of grouped 3D fields (freed with slot 0)
+    int64_t slot_bytes = 0;     // distance of the slots of a grouped field (bytes)
     int64_t stagger = 4352;
     int nallocs = 0;
     std::string err;
@@ -100,27 +105,50 @@ int fail(pts_ctx* c, int code, const std::string& msg) {
             return fail((ctx), PTS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
     } while (0)
 
-// One field array of nalloc elements of `elem` bytes, zeroed.  With a
+// Elements of the zeroed guard zone before and after each 3D field array:
+// k3_iter2 loads every field at its fixed shift without lane or z-range
+// predicates, so the halo lanes and rows of the edge tiles (a row's worth of
+// columns and up to 32 rows beyond the box) and the planes before the first
+// and after the last read inside the allocation.  Those values only feed
+// results that are not stored.
+int64_t field_guard_elems(const pts_ctx* c) {
+    return c->nd == 3 ? (int64_t)c->sxy + 32 * (int64_t)c->sx + 256 : 0;
+}
+
+size_t field_guard_bytes(const pts_ctx* c, size_t elem) {
+    return ((size_t)field_guard_elems(c) * elem + 255) / 256 * 256;
+}
+
+// Distance of the slots of a grouped field: the array and one guard zone,
+// plus one stagger step so that the slots do not align with each other.
+size_t slot_stride(const pts_ctx* c, size_t elem) {
+    return ((size_t)c->nalloc * elem + field_guard_bytes(c, elem) + 255) / 256 * 256 + (size_t)c->stagger;
+}
+
+// One field array of nalloc elements of `elem` bytes, zeroed (with `slots`
+// > 1: that many arrays, slot_stride apart, in one allocation).  With a
 // stagger of S bytes (a multiple of 256), the n-th array starts (n mod 16)·S
 // bytes into its block, so equal offsets of different arrays do not all fall
 // on the same HBM channel when the array size is a power of two.
-int field_alloc(pts_ctx* c, size_t elem, void** out) {
+int field_alloc(pts_ctx* c, size_t elem, void** out, int slots = 1) {
     const size_t bytes = (size_t)c->nalloc * elem;
-    const size_t off = (size_t)((c->nallocs % 16) * c->stagger);
+    const size_t guard = field_guard_bytes(c, elem);
+    const size_t off = guard + (size_t)((c->nallocs % 16) * c->stagger);
+    const size_t total = bytes + 16 * (size_t)c->stagger + 2 * guard + (size_t)(slots - 1) * slot_stride(c, elem);
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes + 16 * (size_t)c->stagger);
+    hipError_t e = hipMalloc(&p, total);
     if (e != hipSuccess) return fail(c, PTS_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
     void* q = (char*)p + off;
-    HIPCHK(c, hipMemsetAsync(q, 0, bytes, c->stream));
+    HIPCHK(c, hipMemsetAsync(p, 0, total, c->stream));
     c->alloc_base[q] = p;
     c->nallocs += 1;
-    c->bytes += (int64_t)(bytes + 16 * (size_t)c->stagger);
+    c->bytes += (int64_t)total;
     *out = q;
     return PTS_OK;
 }
 
 void field_free(pts_ctx* c, void* q) {
-    if (!q) return;
+    if (!q || c->slot_ptrs.count(q)) return;
     auto it = c->alloc_base.find(q);
     (void)hipFree(it != c->alloc_base.end() ? it->second : q);
     if (it != c->alloc_base.end()) c->alloc_base.erase(it);
@@ -140,9 +168,39 @@ bool field_valid(const pts_ctx* c, int f) {
     return true;
 }
 
+// 3D state fields are allocated as groups: slot 0 the field, slot 1 its
+// ping-pong partner and, for the stresses of a visco-elastic context, slot 2
+// the old-time stress τ_o, so that the single-pass kernel addresses all of
+// them from one base pointer (k3_iter2).
+int ensure_group(pts_ctx* c, int f) {
+    const bool stress = f >= PTS_TXX && f <= PTS_TYZ;
+    const int slots = (stress && c->g.ve) ? 3 : 2;
+    void* p = nullptr;
+    if (int rc = field_alloc(c, sizeof(double), &p, slots)) return rc;
+    const size_t sb = slot_stride(c, sizeof(double));
+    c->slot_bytes = (int64_t)sb;
+    char* q = (char*)p;
+    c->cur[f] = (double*)q;
+    c->alt[f] = (double*)(q + sb);
+    c->slot_ptrs.insert(c->alt[f]);
+    c->alt_synced = false;
+    if (slots == 3) {
+        c->cur[f + (PTS_TXX_O - PTS_TXX)] = (double*)(q + 2 * sb);
+        c->slot_ptrs.insert(q + 2 * sb);
+    }
+    return PTS_OK;
+}
+
 int ensure(pts_ctx* c, int f) {
     if (!field_valid(c, f)) return fail(c, PTS_ERR_STATE, "field " + std::to_string(f) + " not valid for this grid");
     if (c->cur[f]) return PTS_OK;
+    if (c->nd == 3 && c->group_3d) {
+        if (f >= PTS_PT && f <= PTS_TYZ) return ensure_group(c, f);
+        if (f >= PTS_TXX_O && f <= PTS_TYZ_O && c->g.ve) {
+            const int s = f - (PTS_TXX_O - PTS_TXX);
+            if (!c->cur[s]) return ensure_group(c, s);
+        }
+    }
     void* p = nullptr;
     if (int rc = field_alloc(c, sizeof(double), &p)) return rc;
     c->cur[f] = (double*)p;
@@ -262,7 +320,10 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     c->ny = g->ny;
     c->nz = nz;
     c->sx = ((g->nx + 1) + 31) / 32 * 32;
-    const int64_t sxy = (int64_t)c->sx * (g->ny + 1);
+    int64_t sxy = (int64_t)c->sx * (g->ny + 1);
+    // 3D: optional padding of the plane stride (elements, multiple of 32)
+    if (g->ndim == 3)
+        if (const char* e = std::getenv("PTS_PLANE_PAD")) sxy += std::max(0, std::atoi(e)) / 32 * 32;
     const int64_t na = sxy * (g->ndim == 3 ? nz + 1 : 1);
     if (na >= (int64_t)1 << 31) {
         delete c;
@@ -276,6 +337,8 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
+    if (const char* e = std::getenv("PTS_3D_VARIANT")) c->variant_3d = std::atoi(e);
+    if (const char* e = std::getenv("PTS_3D_GROUPS")) c->group_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_DERIVED_PARAMS")) c->dp_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_DERIVED_MUSTAU")) c->mx_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_ADJ_TWO_PASS")) c->adj_fused = std::atoi(e) == 0;
@@ -779,7 +842,7 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s0, int64_t n, bool stor
                                        c->variant_2d, c->persist_2d, dp, use_mx(c)));
             else
                 HIPCHK(c, launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
-                                       c->tile_rows_3d, dp));
+                                       c->tile_rows_3d, dp, c->variant_3d, c->slot_bytes));
             swap_state(c);
         } else {
             if (store_last_dv && q == n - 1 && !mul) {

@@ -52,16 +52,17 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return x * q + min(x, rem) + i;
 }
 
+// bound_ctrl: lanes without a source lane write 0 (no old value to set up).
 __device__ __forceinline__ double from_left(double v) {
     int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x138, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x138, 0xf, 0xf, true);
     return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double from_right(double v) {
     int lo = __double2loint(v), hi = __double2hiint(v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, false);
+    lo = __builtin_amdgcn_mov_dpp(lo, 0x130, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
     return __hiloint2double(hi, lo);
 }
 
@@ -94,17 +95,22 @@ __device__ __forceinline__ double tau_visc2d_div(double t, double g, double m, d
 
 // Write v at (a,b) on the tangential plane of a velocity component and at the
 // free-slip copies it feeds: index a (stride sa, extent na) and b (stride sb,
-// extent nb); a==1 also writes a=0, a==na-2 also writes a=na-1, same for b
+// extent nb); a==1 also writes a=0, a==na-2 also writes a=na-1 (both when
+// na == 3: the one interior plane feeds both boundary planes), same for b
 // (bc_y!/bc_z! etc., Stokes3D_ve_perf.jl:202-207; edge values end up equal to
 // the diagonal interior neighbour whatever the order of the copies).
 template <typename T>
 __device__ __forceinline__ void put_bc(T* V, int q, T v, int a, int na, int sa, int b, int nb, int sb) {
     V[q] = v;
-    const int da = (a == 1) ? -1 : ((a == na - 2) ? 1 : 0);
-    const int db = (b == 1) ? -1 : ((b == nb - 2) ? 1 : 0);
-    if (da) V[q + da * sa] = v;
-    if (db) V[q + db * sb] = v;
-    if (da && db) V[q + da * sa + db * sb] = v;
+    const bool alo = a == 1, ahi = a == na - 2, blo = b == 1, bhi = b == nb - 2;
+    if (alo) V[q - sa] = v;
+    if (ahi) V[q + sa] = v;
+    if (blo) V[q - sb] = v;
+    if (bhi) V[q + sb] = v;
+    if (alo && blo) V[q - sa - sb] = v;
+    if (alo && bhi) V[q - sa + sb] = v;
+    if (ahi && blo) V[q + sa - sb] = v;
+    if (ahi && bhi) V[q + sa + sb] = v;
 }
 
 // Scalars of one PT iteration.  dp kernels (`dp` launch flag) read Musτ and
@@ -116,6 +122,8 @@ struct StepScalars {
     // dτ_Rho from Musτ instead of reading them (dp_ve: visco-elastic μ_eff)
     int dp_ve;
     double ip_igdt, ip_k1, ip_k2, ip_r2;
+    // 1/(G·dt) when G·dt is a power of two (then Gdτ·iGdt == Gdτ/(G·dt)), else 0
+    double iGdt;
 };
 
 // dτ_Rho and Gdτ of one cell from Musτ, operation for operation as
@@ -177,8 +185,13 @@ hipError_t launch3_compute_Pt_tau(hipStream_t s, const Fields& f, const StepScal
 hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScalars& c, bool bf, bool mul,
                                   bool fold_bc);
 // 3D single-pass iteration (ping-pong)
+// variant 0: k3_iter; 1: k3_iter2; 2: k3_iter2 with the next plane's loads
+// issued before the current plane's arithmetic.  k3_iter2 needs the grouped
+// 3D allocations (slot_bytes apart, with guard zones: see ensure_group and
+// field_guard_elems in pts_api.hip); otherwise k3_iter runs.
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
-                        bool mul, bool bf, int planes_per_block, int tile_rows, bool dp);
+                        bool mul, bool bf, int planes_per_block, int tile_rows, bool dp, int variant,
+                        int64_t slot_bytes);
 
 // Mixed-precision defect correction (pts_dc.hip)
 hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim);

@@ -13,6 +13,8 @@
 // plane), y-neighbours of read-only inputs from the cache, z-neighbours from
 // registers.  State is read from `in` and written to `out` (ping-pong); the
 // never-updated Dirichlet entries are held by both buffers.
+#include <cmath>
+#include <cstdint>
 #include <type_traits>
 
 #include "pts_internal.hpp"
@@ -303,6 +305,28 @@ __global__ __launch_bounds__(64 * BY) void k3_iter(const Fields in, const Fields
     }
 }
 
+// Planes per block (Lz > 0: Lz).  Lz < 0: the tiles of a z-chunk start
+// together and march to its end, so a launch takes ceil(blocks / resident)
+// rounds of about L + 3 plane steps (the redundant plane below the chunk and
+// the start-up of a block); take the L in [4, -Lz] with the fewest such steps
+// (ties: the longer march).  At 511^3 (468 tiles, 256 resident) that is 86
+// planes: 2808 blocks in 11 nearly full rounds.
+int planes_per_block(int Lz, int nz, int64_t ntile, int64_t conc) {
+    if (Lz > 0) return Lz;
+    const int lmax = std::max(1, std::min(-Lz, nz));
+    int L = lmax;
+    int64_t best = -1;
+    for (int l = lmax; l >= std::min(4, lmax); --l) {
+        const int64_t nch = (nz + l - 1) / l;
+        const int64_t cost = (ntile * nch + conc - 1) / conc * (l + 3);
+        if (best < 0 || cost < best) {
+            best = cost;
+            L = l;
+        }
+    }
+    return L;
+}
+
 // Resident blocks of one k3_iter instance on the whole device.
 template <bool VE, bool MUL, bool BF, int BY, bool DP>
 int concurrency3() {
@@ -317,20 +341,14 @@ int concurrency3() {
     return cached;
 }
 
-// Lz planes per block; Lz < 0: about one round of resident blocks, within
-// [4, -Lz] planes (small grids get short columns, large ones long marches).
+// Lz planes per block (Lz < 0: planes_per_block's choice within [4, -Lz]).
 template <bool VE, bool MUL, bool BF, bool DP>
 void go3(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int Lz, int by) {
     const int ntx = (in.nx + 1 + 61) / 62;
     auto launch = [&](auto byc) {
         constexpr int BY = decltype(byc)::value;
         const int nty = (in.ny + 1 + (BY - 2) - 1) / (BY - 2);
-        int L = Lz;
-        if (L < 0) {
-            const int nch = std::max(1, concurrency3<VE, MUL, BF, BY, DP>() / (ntx * nty));
-            L = std::max(4, std::min((in.nz + nch - 1) / nch, -Lz));
-        }
-        L = std::max(1, L);
+        const int L = planes_per_block(Lz, in.nz, (int64_t)ntx * nty, concurrency3<VE, MUL, BF, BY, DP>());
         const int nch = (in.nz + L - 1) / L;
         k3_iter<VE, MUL, BF, BY, DP><<<ntx * nty * nch, dim3(64, BY), 0, s>>>(in, out, c, L, ntx, nty);
     };
@@ -343,11 +361,515 @@ void go3(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& 
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// k3_iter2 -- the same iteration and traversal, with less work per cell:
+//   * one 32-bit byte offset per thread addresses every load and store: each
+//     field's pointer is moved to the current plane and carries the field's
+//     fixed index shift (txx at (c, d-1, k-1), edges at (c-1, d-1, k-1), ...),
+//     so the loads use the scalar-base form and need no per-array address;
+//   * loads carry no lane predicates: 3D field allocations have guard zones
+//     around the strided box that absorb the halo lanes and rows beyond the
+//     extents; only the z-ranges (uniform per plane) gate loads.  Values read
+//     outside a field's extent only feed lanes whose results are not stored;
+//   * the row-above inputs (Gdτ, dτ_Rho, μ, Vx, Vz) and the row-below Vy come
+//     from LDS, written by the neighbouring row one plane ahead, instead of
+//     being loaded again and (Gdτ, dτ_Rho) recomputed from Musτ;
+//   * the halo rows run only what their neighbour row needs (the top row its
+//     Pt and τyy, the bottom row its τxy and τyz), on wave-uniform branches;
+//   * PF: the loads of plane k+1 are issued before the arithmetic of plane k;
+//   * G2: with G·dt a power of two, τ·Gr uses Gdτ·(1/(G·dt)), which rounds
+//     exactly as Gdτ/(G·dt).
+// Bit-identical to k3_iter.
+
+// Fields of k3_iter2.  Each state field's input and output buffers (and,
+// for the visco-elastic stresses, τ_o) are slots of one allocation: base
+// points at slot 0 and the slots are byte offsets from it.
+struct Fields3g {
+    const double* base[PTS_TYZ + 1];  // PT, VX, VY, VZ, TXX .. TYZ
+    const double* g;                  // Musτ (dp) or Gdτ
+    const double* mu;
+    const double* rh;                 // dτ_Rho (without dp)
+    const double* rg;                 // ρg (body force)
+    uint32_t off_in, off_out, off_old;
+    int sx, sxy, nx, ny, nz;
+};
+
+struct Plane3 {  // inputs of one plane at (c, d); g/h: Gdτ, dτ_Rho after the exchange
+    double g, h, mu, pt, vx, vy, vzn, rg;
+    double tx, txo, ty, tyo, tz, tzo, txy, txyo, txz, txzo, tyz, tyzo;
+};
+struct Nbr3 {  // the same plane from the neighbouring rows
+    double gU, rhU, muU, vxU, vzUN, vyN;
+};
+struct Carry3 {  // plane k-1
+    double vxP, vyP, gP, muP, gLP, muLP, gUP, muUP, rhP, rhUP, rgP;
+    double pt1P, txx1P, tyy1P, tzz1P, txy1P, txz1P, tyz1P, pt1PU, tyy1PU, txy1PD;
+    double vzC, vzUC;  // Vz at z-face k (own row, row above)
+};
+
+__device__ __forceinline__ double ldb(const double* p, uint32_t boff) {
+    // an empty asm makes the offset a value of the load's own block, so that
+    // the address is formed there as scalar base + 32-bit offset (no 64-bit
+    // per-lane address) also for loads under branches
+    asm volatile("" : "+v"(boff));
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + boff);
+}
+__device__ __forceinline__ void stb(double* p, uint32_t boff, double v) {
+    *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + boff) = v;
+}
+// put_bc with a byte offset (same copies)
+__device__ __forceinline__ void put_bcb(double* V, uint32_t q, double v, int a, int na, int sa, int b, int nb, int sb) {
+    stb(V, q, v);
+    const bool alo = a == 1, ahi = a == na - 2, blo = b == 1, bhi = b == nb - 2;
+    const uint32_t ea = (uint32_t)sa * 8u, eb = (uint32_t)sb * 8u;
+    if (alo) stb(V, q - ea, v);
+    if (ahi) stb(V, q + ea, v);
+    if (blo) stb(V, q - eb, v);
+    if (bhi) stb(V, q + eb, v);
+    if (alo && blo) stb(V, q - ea - eb, v);
+    if (alo && bhi) stb(V, q - ea + eb, v);
+    if (ahi && blo) stb(V, q + ea - eb, v);
+    if (ahi && bhi) stb(V, q + ea + eb, v);
+}
+
+template <bool VE, bool G2>
+__device__ __forceinline__ double tau_upd2(double t, double to, double g, double m, const StepScalars& cs, double e) {
+    if constexpr (VE) {
+        const double Gr = G2 ? g * cs.iGdt : g / cs.Gdt;
+        return (t + to * Gr + 2.0 * g * e) / (1.0 + g / m + Gr);
+    } else {
+        return (t + 2.0 * g * e) / (1.0 + g / m);
+    }
+}
+
+template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2>
+__global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const StepScalars cs, int Lz,
+                                                    int ntx, int nty, int nchunk) {
+    __shared__ double shA[4][BY][64];  // Pt1, τyy1 (read by row +1); τxy1, τyz1 (read by row -1)
+    __shared__ double shB[7][BY][64];  // next-plane inputs: Gdτ, dτ_Rho, μ, Vx, Vz (row +1), Vy (row -1); Vz (start)
+
+    // Block b runs on XCD b % 8.  Each XCD owns a contiguous eighth of the
+    // tiles (neighbouring tiles share halo rows in its L2) and takes the
+    // z-chunks in order, so all XCDs sweep the same chunk at the same time on
+    // different rows of the planes (rather than eight chunks whose planes lie
+    // a multiple of the plane size apart).  Blocks beyond an XCD's share exit.
+    const int ntile = ntx * nty;
+    const int xcd = blockIdx.x & 7, seq = blockIdx.x >> 3;
+    const int lo = (int)((int64_t)ntile * xcd / 8), cnt = (int)((int64_t)ntile * (xcd + 1) / 8) - lo;
+    if (seq >= cnt * nchunk) return;
+    const int tile = lo + seq % cnt, chunk = seq / cnt;
+    const int tx = tile % ntx, ty = tile / ntx;
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int c = tx * 62 - 1 + lane;
+    const int d = ty * (BY - 2) - 1 + w;
+    const int nx = f.nx, ny = f.ny, nz = f.nz, sx = f.sx, sxy = f.sxy;
+    const int kb = chunk * Lz;
+    const int ke = min(kb + Lz, nz);
+    // byte offset of (c, d) within a plane, biased so that c, d >= -1 stay >= 0
+    const int bias = sx + 1;
+    const uint32_t tb = (uint32_t)(c + sx * d + bias) * 8u;
+
+
+    // element shifts of the fields relative to (c, d, k)
+    const long long sTXX = -(long long)sx - sxy, sTYY = -1 - (long long)sxy, sTZZ = -1 - (long long)sx;
+    const long long sE = -1 - (long long)sx - sxy;
+    // one pointer per field (slot 0 of its allocation, moved by the field's
+    // shift and the bias); the input, output and τ_o slots are byte offsets
+    const double* __restrict__ Ptb = f.base[PTS_PT] - bias;
+    const double* __restrict__ txxb = f.base[PTS_TXX] + sTXX - bias;
+    const double* __restrict__ tyyb = f.base[PTS_TYY] + sTYY - bias;
+    const double* __restrict__ tzzb = f.base[PTS_TZZ] + sTZZ - bias;
+    const double* __restrict__ txyb = f.base[PTS_TXY] + sE - bias;
+    const double* __restrict__ txzb = f.base[PTS_TXZ] + sE - bias;
+    const double* __restrict__ tyzb = f.base[PTS_TYZ] + sE - bias;
+    const double* __restrict__ Vxb = f.base[PTS_VX] - bias;
+    const double* __restrict__ Vyb = f.base[PTS_VY] - bias;
+    const double* __restrict__ Vzb = f.base[PTS_VZ] - bias;
+    const double* __restrict__ Gp = f.g - bias;
+    const double* __restrict__ Mup = f.mu - bias;
+    const double* __restrict__ Rhp = f.rh - bias;
+    const double* __restrict__ rgp = f.rg - bias;
+    const uint32_t oin = f.off_in, oout = f.off_out, oold = f.off_old;
+
+    double* __restrict__ Ptm = const_cast<double*>(Ptb);
+    double* __restrict__ txxm = const_cast<double*>(txxb);
+    double* __restrict__ tyym = const_cast<double*>(tyyb);
+    double* __restrict__ tzzm = const_cast<double*>(tzzb);
+    double* __restrict__ txym = const_cast<double*>(txyb);
+    double* __restrict__ txzm = const_cast<double*>(txzb);
+    double* __restrict__ tyzm = const_cast<double*>(tyzb);
+    double* __restrict__ Vxm = const_cast<double*>(Vxb);
+    double* __restrict__ Vym = const_cast<double*>(Vyb);
+    double* __restrict__ Vzm = const_cast<double*>(Vzb);
+
+    const double dx = cs.dx, dy = cs.dy, dz = cs.dz, _dx = cs._dx, _dy = cs._dy, _dz = cs._dz;
+    const double r = cs.r;
+    const int k0 = max(kb - 1, 0), k1 = min(ke, nz - 1);
+    // byte offset of (c, d, k): one 32-bit value per plane for every field
+    auto tko = [&](int k) __attribute__((always_inline)) { return tb + (uint32_t)k * (uint32_t)sxy * 8u; };
+
+    const uint32_t sxy8 = (uint32_t)sxy * 8u;
+
+    // Each row role runs its own loop (R = 0: top halo row, 1: own rows,
+    // 2: bottom halo row; w is wave-uniform), loading only what it uses.  The
+    // loads need no z-range conditions either: the guard zones hold a plane.
+    auto load = [&](auto role, int k, Plane3& I) __attribute__((always_inline)) {
+        constexpr int R = decltype(role)::value;
+        const uint32_t t = tko(k), ti = t + oin, to = t + oold;
+        I.g = ldb(Gp, t);
+        if constexpr (!DP && R != 2) I.h = ldb(Rhp, t);
+        I.mu = ldb(Mup, t);
+        I.vx = ldb(Vxb, ti);
+        I.vy = ldb(Vyb, ti);
+        I.vzn = ldb(Vzb, ti + sxy8);
+        if constexpr (R != 2) I.pt = ldb(Ptb, ti);
+        if constexpr (BF && R == 1) I.rg = ldb(rgp, t);
+        if constexpr (R == 1) {
+            I.tx = ldb(txxb, ti);
+            if constexpr (VE) I.txo = ldb(txxb, to);
+            I.tz = ldb(tzzb, ti);
+            if constexpr (VE) I.tzo = ldb(tzzb, to);
+            I.txz = ldb(txzb, ti);
+            if constexpr (VE) I.txzo = ldb(txzb, to);
+        }
+        if constexpr (R != 2) {
+            I.ty = ldb(tyyb, ti);
+            if constexpr (VE) I.tyo = ldb(tyyb, to);
+        }
+        if constexpr (R != 0) {
+            I.txy = ldb(txyb, ti);
+            if constexpr (VE) I.txyo = ldb(txyb, to);
+            I.tyz = ldb(tyzb, ti);
+            if constexpr (VE) I.tyzo = ldb(tyzb, to);
+        }
+    };
+    // Gdτ, dτ_Rho of the plane (DP: from Musτ) and the exchange with the
+    // neighbouring rows; `start`: also pass Vz at the first face
+    auto exchange = [&](auto role, Plane3& I, Nbr3& U, bool start, double vzc, double& vzUC)
+                        __attribute__((always_inline)) {
+        constexpr int R = decltype(role)::value;
+        if constexpr (DP) {
+            double rh, gd;
+            iter_params_of(I.g, cs, rh, gd);
+            I.g = gd;
+            I.h = rh;
+        }
+        if constexpr (R != 2) {  // read by the row below
+            shB[0][w][lane] = I.g;
+            shB[1][w][lane] = I.h;
+            shB[2][w][lane] = I.mu;
+            shB[3][w][lane] = I.vx;
+            shB[4][w][lane] = I.vzn;
+            if (start) shB[6][w][lane] = vzc;
+        }
+        if constexpr (R != 0) shB[5][w][lane] = I.vy;  // read by the row above
+        __syncthreads();
+        if constexpr (R != 0) {
+            U.gU = shB[0][w - 1][lane];
+            U.rhU = shB[1][w - 1][lane];
+            U.muU = shB[2][w - 1][lane];
+            U.vxU = shB[3][w - 1][lane];
+            U.vzUN = shB[4][w - 1][lane];
+            if (start) vzUC = shB[6][w - 1][lane];
+        }
+        if constexpr (R != 2) U.vyN = shB[5][w + 1][lane];
+    };
+
+    auto step = [&](auto role, int k, Plane3& I, Plane3& N, Nbr3& U, Carry3& P) __attribute__((always_inline)) {
+        constexpr int R = decltype(role)::value;
+        if constexpr (PF) {
+            if (k < k1) load(role, k + 1, N);
+        }
+        const double Gd = I.g, Rh = I.h, Mu = I.mu;
+        double Pt1n = 0.0, txx1n = 0.0, tyy1n = 0.0, tzz1n = 0.0, txy1n = 0.0, txz1n = 0.0, tyz1n = 0.0;
+        double GdL = 0.0, MuL = 0.0;
+        if constexpr (R != 2) {  // cell (c, d, k)
+            const double VxR = from_right(I.vx);
+            const double ex = dfd<MUL>(VxR - I.vx, dx, _dx);
+            const double ey = dfd<MUL>(U.vyN - I.vy, dy, _dy);
+            const double ez = dfd<MUL>(I.vzn - P.vzC, dz, _dz);
+            Pt1n = I.pt - r * Gd * (ex + ey + ez);
+            tyy1n = tau_upd2<VE, G2>(I.ty, I.tyo, Gd, Mu, cs, ey);
+            if constexpr (R == 1) {
+                txx1n = tau_upd2<VE, G2>(I.tx, I.txo, Gd, Mu, cs, ex);
+                tzz1n = tau_upd2<VE, G2>(I.tz, I.tzo, Gd, Mu, cs, ez);
+            }
+        }
+        if constexpr (R != 0) {  // edges
+            GdL = from_left(Gd);
+            MuL = from_left(Mu);
+            const double GdUL = from_left(U.gU), MuUL = from_left(U.muU);
+            const double VyL = from_left(I.vy);
+            {   // τxy at (x-face c, y-face d, cell k)
+                const double ag = (GdUL + U.gU + GdL + Gd) * 0.25;
+                const double am = (MuUL + U.muU + MuL + Mu) * 0.25;
+                const double e = 0.5 * (dfd<MUL>(I.vx - U.vxU, dy, _dy) + dfd<MUL>(I.vy - VyL, dx, _dx));
+                txy1n = tau_upd2<VE, G2>(I.txy, I.txyo, ag, am, cs, e);
+            }
+            if constexpr (R == 1) {  // τxz at (x-face c, cell d, z-face k)
+                const double VzL = from_left(P.vzC);
+                const double ag = (P.gLP + P.gP + GdL + Gd) * 0.25;
+                const double am = (P.muLP + P.muP + MuL + Mu) * 0.25;
+                const double e = 0.5 * (dfd<MUL>(I.vx - P.vxP, dz, _dz) + dfd<MUL>(P.vzC - VzL, dx, _dx));
+                txz1n = tau_upd2<VE, G2>(I.txz, I.txzo, ag, am, cs, e);
+            }
+            {   // τyz at (cell c, y-face d, z-face k)
+                const double ag = (P.gUP + P.gP + U.gU + Gd) * 0.25;
+                const double am = (P.muUP + P.muP + U.muU + Mu) * 0.25;
+                const double e = 0.5 * (dfd<MUL>(I.vy - P.vyP, dz, _dz) + dfd<MUL>(P.vzC - P.vzUC, dy, _dy));
+                tyz1n = tau_upd2<VE, G2>(I.tyz, I.tyzo, ag, am, cs, e);
+            }
+        }
+
+        // y-neighbour exchange of the new values
+        if constexpr (R != 2) {
+            shA[0][w][lane] = Pt1n;
+            shA[1][w][lane] = tyy1n;
+        }
+        if constexpr (R != 0) {
+            shA[2][w][lane] = txy1n;
+            shA[3][w][lane] = tyz1n;
+        }
+        __syncthreads();
+        double Pt1U = 0.0, tyy1U = 0.0, txy1D = 0.0;
+        if constexpr (R == 1) {
+            Pt1U = shA[0][w - 1][lane];   // (c, d-1, k)
+            tyy1U = shA[1][w - 1][lane];
+            txy1D = shA[2][w + 1][lane];  // τxy (c, y-face d+1, k)
+            const double tyz1D = shA[3][w + 1][lane];  // τyz (c, y-face d+1, z-face k)
+
+            const uint32_t t = tko(k) + oout;
+            const double txz1R = from_right(txz1n);
+            const bool kin2 = k >= 1 && k <= nz - 2, kin1 = k >= 1 && k <= nz - 1;
+            // Store predicates: a lane part from the column and a wave-uniform
+            // part from the row.  The column goes through an empty asm so that
+            // the masks are formed here each step (a few compares) instead of
+            // being held in scalar registers across the loop.
+            int cl = c;
+            asm volatile("" : "+v"(cl));
+            const bool ol = lane >= 1 && lane <= 62;
+            const bool l_cell = ol && cl < nx;                 // cells, τxx
+            const bool l_in2 = ol && cl >= 1 && cl <= nx - 2;  // τyy, τzz, τyz, Vy, Vz
+            const bool l_in1 = ol && cl >= 1 && cl <= nx - 1;  // τxy, τxz, Vx
+            const bool l_dir = ol && (cl == 0 || cl == nx);    // Dirichlet Vx faces
+            const bool r_in = d >= 1 && d <= ny - 2, r_in1 = d >= 1 && d <= ny - 1;
+            const bool s_cell = l_cell && d < ny, s_txx = l_cell && r_in, s_tyy = l_in2 && d <= ny - 1;
+            const bool s_tzz = l_in2 && r_in, s_txy = l_in1 && r_in1, s_txz = l_in1 && r_in, s_tyz = l_in2 && r_in1;
+            const bool vx_in = l_in1 && r_in, vx_dir = l_dir && r_in;
+            const bool vy_in = l_in2 && r_in1, vy_dir = l_in2 && (d == 0 || d == ny);
+            const bool vz_in = l_in2 && r_in;
+            if (k >= kb && k < ke) {
+                if (s_cell) stb(Ptm, t, Pt1n);
+                if (kin2 && s_txx) stb(txxm, t, txx1n);
+                if (kin2 && s_tyy) stb(tyym, t, tyy1n);
+                if (s_tzz) stb(tzzm, t, tzz1n);
+                if (kin2 && s_txy) stb(txym, t, txy1n);
+                if (kin1 && s_txz) stb(txzm, t, txz1n);
+                if (kin1 && s_tyz) stb(tyzm, t, tyz1n);
+                // Vz at z-face k
+                if (vz_in) {
+                    if (k >= 1) {
+                        double Rz = dfd<MUL>(tzz1n - P.tzz1P, dz, _dz) + dfd<MUL>(txz1R - txz1n, dx, _dx) +
+                                    dfd<MUL>(tyz1D - tyz1n, dy, _dy) - dfd<MUL>(Pt1n - P.pt1P, dz, _dz);
+                        if constexpr (BF) Rz = Rz - (P.rgP + I.rg) * 0.5;
+                        put_bcb(Vzm, t, P.vzC + Rz * ((P.rhP + Rh) * 0.5), cl, nx, 1, d, ny, sx);
+                    } else {  // Dirichlet face z = 0
+                        put_bcb(Vzm, t, P.vzC, cl, nx, 1, d, ny, sx);
+                    }
+                    if (k == nz - 1) put_bcb(Vzm, t + sxy8, I.vzn, cl, nx, 1, d, ny, sx);  // face z = nz
+                }
+            }
+            // Vx, Vy at plane k-1
+            const int km = k - 1;
+            if (km >= kb && km < ke && km >= 1 && km <= nz - 2) {
+                const double Pt1PL = from_left(P.pt1P), txx1PL = from_left(P.txx1P), RhPL = from_left(P.rhP);
+                const double txy1PR = from_right(P.txy1P);
+                const uint32_t tq = t - sxy8;
+                if (vx_in) {
+                    const double Rx = dfd<MUL>(P.txx1P - txx1PL, dx, _dx) + dfd<MUL>(P.txy1PD - P.txy1P, dy, _dy) +
+                                      dfd<MUL>(txz1n - P.txz1P, dz, _dz) - dfd<MUL>(P.pt1P - Pt1PL, dx, _dx);
+                    put_bcb(Vxm, tq, P.vxP + Rx * ((RhPL + P.rhP) * 0.5), d, ny, sx, km, nz, sxy);
+                } else if (vx_dir) {
+                    put_bcb(Vxm, tq, P.vxP, d, ny, sx, km, nz, sxy);
+                }
+                if (vy_in) {
+                    const double Ry = dfd<MUL>(P.tyy1P - P.tyy1PU, dy, _dy) + dfd<MUL>(txy1PR - P.txy1P, dx, _dx) +
+                                      dfd<MUL>(tyz1n - P.tyz1P, dz, _dz) - dfd<MUL>(P.pt1P - P.pt1PU, dy, _dy);
+                    put_bcb(Vym, tq, P.vyP + Ry * ((P.rhUP + P.rhP) * 0.5), cl, nx, 1, km, nz, sxy);
+                } else if (vy_dir) {
+                    put_bcb(Vym, tq, P.vyP, cl, nx, 1, km, nz, sxy);
+                }
+            }
+        }
+
+        // carry to plane k+1 (what the role uses)
+        P.vzC = I.vzn;
+        if constexpr (R != 0) {
+            P.vyP = I.vy;
+            P.vzUC = U.vzUN;
+            P.gUP = U.gU;
+            P.muUP = U.muU;
+            P.gP = Gd;
+            P.muP = Mu;
+        }
+        if constexpr (R == 1) {
+            P.vxP = I.vx;
+            P.gLP = GdL;
+            P.muLP = MuL;
+            P.rhUP = U.rhU;
+            P.rhP = Rh;
+            if constexpr (BF) P.rgP = I.rg;
+            P.pt1P = Pt1n;
+            P.txx1P = txx1n;
+            P.tyy1P = tyy1n;
+            P.tzz1P = tzz1n;
+            P.txy1P = txy1n;
+            P.txz1P = txz1n;
+            P.tyz1P = tyz1n;
+            P.pt1PU = Pt1U;
+            P.tyy1PU = tyy1U;
+            P.txy1PD = txy1D;
+        }
+
+        if (k < k1) {
+            if constexpr (!PF) load(role, k + 1, N);
+            double unused = 0.0;
+            exchange(role, N, U, false, 0.0, unused);
+        }
+    };
+
+    auto run = [&](auto role) __attribute__((always_inline)) {
+        Plane3 A{}, B{};
+        Nbr3 U{};
+        Carry3 P{};
+        load(role, k0, A);
+        const double vz0 = ldb(Vzb, tko(k0) + oin);  // Vz at z-face k0
+        P.vzC = vz0;
+        exchange(role, A, U, true, vz0, P.vzUC);
+        for (int k = k0;;) {
+            step(role, k, A, B, U, P);
+            if (k == k1) break;
+            ++k;
+            step(role, k, B, A, U, P);
+            if (k == k1) break;
+            ++k;
+        }
+    };
+    if (w == 0) run(std::integral_constant<int, 0>{});
+    else if (w == BY - 1) run(std::integral_constant<int, 2>{});
+    else run(std::integral_constant<int, 1>{});
+}
+
+template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2>
+int concurrency3b() {
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k3_iter2<VE, MUL, BF, BY, DP, PF, G2>, 64 * BY, 0);
+        cached = std::max(1, cus) * std::max(1, per);
+    }
+    return cached;
+}
+
+template <bool VE, bool MUL, bool BF, bool DP, bool PF, bool G2>
+void go3b(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz) {
+    constexpr int BY = 12;
+    const int ntx = (in.nx + 1 + 61) / 62;
+    const int nty = (in.ny + 1 + (BY - 2) - 1) / (BY - 2);
+    // per XCD: ceil(tiles / 8) tiles per chunk on an eighth of the resident blocks
+    const int64_t per = ((int64_t)ntx * nty + 7) / 8;
+    const int L = planes_per_block(Lz, in.nz, per, std::max<int64_t>(1, concurrency3b<VE, MUL, BF, BY, DP, PF, G2>() / 8));
+    const int nch = (in.nz + L - 1) / L;
+    k3_iter2<VE, MUL, BF, BY, DP, PF, G2><<<(unsigned)(8 * per * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, nch);
+}
+
+template <bool VE, bool MUL, bool BF, bool DP>
+void go3b_pf(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bool pf, bool g2) {
+    if (pf) {
+        if (g2) go3b<VE, MUL, BF, DP, true, true>(s, in, c, Lz);
+        else go3b<VE, MUL, BF, DP, true, false>(s, in, c, Lz);
+    } else {
+        if (g2) go3b<VE, MUL, BF, DP, false, true>(s, in, c, Lz);
+        else go3b<VE, MUL, BF, DP, false, false>(s, in, c, Lz);
+    }
+}
+
+// The grouped view of a ping-pong pair, if its buffers are laid out as slots
+// of one allocation `slot` bytes apart (in and out in slots 0/1 in either
+// order, τ_o in slot 2) and every offset fits in 32 bits.
+bool group_fields(const Fields& in, const Fields& out, bool ve, bool dp, bool bf, int64_t slot, Fields3g& g) {
+    if (slot <= 0) return false;
+    g = Fields3g{};
+    int par = -1;
+    for (int f = PTS_PT; f <= PTS_TYZ; ++f) {
+        const char* a = (const char*)in.p[f];
+        const char* b = (const char*)out.p[f];
+        if (!a || !b) return false;
+        int p;
+        if (b - a == slot) p = 0;
+        else if (a - b == slot) p = 1;
+        else return false;
+        if (par >= 0 && p != par) return false;
+        par = p;
+        const char* base = p ? b : a;
+        g.base[f] = (const double*)base;
+        if (ve && f >= PTS_TXX && (const char*)in.p[f + (PTS_TXX_O - PTS_TXX)] != base + 2 * slot) return false;
+    }
+    g.g = dp ? in.p[PTS_MUSTAU] : in.p[PTS_GDTAU];
+    g.mu = in.p[PTS_MUS];
+    g.rh = in.p[PTS_DTAU_RHO];
+    g.rg = in.p[PTS_RHOG];
+    if (!g.g || !g.mu || (!dp && !g.rh) || (bf && !g.rg)) return false;
+    // largest byte offset: the last slot plus the box, one plane of Vz beyond
+    // and the guard zone (k3_iter2 stays inside the guard zones)
+    const int64_t box = ((int64_t)in.sxy * (in.nz + 1) + 32 * (int64_t)in.sx + 256) * (int64_t)sizeof(double);
+    if ((ve ? 2 : 1) * slot + box >= ((int64_t)1 << 32)) return false;
+    g.off_in = (uint32_t)(par ? slot : 0);
+    g.off_out = (uint32_t)(par ? 0 : slot);
+    g.off_old = (uint32_t)(2 * slot);
+    g.sx = in.sx;
+    g.sxy = in.sxy;
+    g.nx = in.nx;
+    g.ny = in.ny;
+    g.nz = in.nz;
+    return true;
+}
+
 }  // namespace
 
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve, bool mul,
-                        bool bf, int Lz, int by, bool dp) {
+                        bool bf, int Lz, int by, bool dp, int variant, int64_t slot_bytes) {
     if (Lz == 0) Lz = 1;
+    Fields3g g;
+    if ((variant == 1 || variant == 2) && group_fields(in, out, ve, dp, bf, slot_bytes, g)) {
+        const bool pf = variant == 2;
+        int e2 = 0;
+        const bool g2 = ve && std::frexp(c.Gdt, &e2) == 0.5;  // G·dt a power of two
+        StepScalars c2 = c;
+        c2.iGdt = g2 ? 1.0 / c.Gdt : 0.0;
+        if (ve) {
+            if (mul) {
+                if (dp) { if (bf) go3b_pf<true, true, true, true>(s, g, c2, Lz, pf, g2); else go3b_pf<true, true, false, true>(s, g, c2, Lz, pf, g2); }
+                else { if (bf) go3b_pf<true, true, true, false>(s, g, c2, Lz, pf, g2); else go3b_pf<true, true, false, false>(s, g, c2, Lz, pf, g2); }
+            } else {
+                if (dp) { if (bf) go3b_pf<true, false, true, true>(s, g, c2, Lz, pf, g2); else go3b_pf<true, false, false, true>(s, g, c2, Lz, pf, g2); }
+                else { if (bf) go3b_pf<true, false, true, false>(s, g, c2, Lz, pf, g2); else go3b_pf<true, false, false, false>(s, g, c2, Lz, pf, g2); }
+            }
+        } else {
+            if (mul) {
+                if (dp) { if (bf) go3b_pf<false, true, true, true>(s, g, c2, Lz, pf, false); else go3b_pf<false, true, false, true>(s, g, c2, Lz, pf, false); }
+                else { if (bf) go3b_pf<false, true, true, false>(s, g, c2, Lz, pf, false); else go3b_pf<false, true, false, false>(s, g, c2, Lz, pf, false); }
+            } else {
+                if (dp) { if (bf) go3b_pf<false, false, true, true>(s, g, c2, Lz, pf, false); else go3b_pf<false, false, false, true>(s, g, c2, Lz, pf, false); }
+                else { if (bf) go3b_pf<false, false, true, false>(s, g, c2, Lz, pf, false); else go3b_pf<false, false, false, false>(s, g, c2, Lz, pf, false); }
+            }
+        }
+        return hipGetLastError();
+    }
     // dp: derive Gdτ, dτ_Rho from Musτ (tile rows 12 only; the others read them)
     if (dp && (by != 8 && by != 16)) {
         if (ve) {

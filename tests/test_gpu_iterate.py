@@ -169,6 +169,31 @@ VARIANTS = [
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_DERIVED_PARAMS": "0"}),
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_STAGGER_BYTES": "0"}),
     ("Stokes3D_ve_multixpu", dict(n=17), {"PTS_STAGGER_BYTES": "65792"}),
+    # 3D kernels: 0 the predicated k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch;
+    # G·dt = 0.3 (divide) and 0.25 (power of two, multiply by the reciprocal)
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_3D_VARIANT": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_3D_VARIANT": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=125, ny=31, nz=9, dt=0.3), {"PTS_3D_VARIANT": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=125, ny=31, nz=9, dt=0.3), {"PTS_3D_VARIANT": "2"}),
+    ("Stokes3D_ve_perf", dict(nx=61, ny=42, nz=11, G=0.5, dt=0.5), {"PTS_3D_VARIANT": "2"}),
+    ("Stokes3D_ve_multixpu", dict(n=19, rhog_in=-1.0), {"PTS_3D_VARIANT": "1"}),
+    ("Stokes3D_ve_multixpu", dict(nx=33, ny=20, nz=27, rhog_in=-1.0), {"PTS_3D_VARIANT": "2"}),
+    ("Stokes3D_ve_perf", dict(nx=62, ny=10, nz=30, ve=False), {"PTS_3D_VARIANT": "2"}),
+    ("Stokes3D_ve_perf", dict(nx=63, ny=11, nz=30, ve=False, rhog_in=-1.0), {"PTS_3D_VARIANT": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_3D_VARIANT": "2", "PTS_DERIVED_PARAMS": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_3D_VARIANT": "1", "PTS_DERIVED_PARAMS": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=20, ny=17, nz=19), {"PTS_3D_VARIANT": "2", "PTS_PLANES_PER_BLOCK": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=20, ny=17, nz=19), {"PTS_3D_VARIANT": "2", "PTS_PLANES_PER_BLOCK": "2"}),
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=80), {"PTS_3D_VARIANT": "2", "PTS_PLANES_PER_BLOCK": "64"}),
+    # extent 3 in z / y: the single interior plane feeds both free-slip copies
+    ("Stokes3D_ve_perf", dict(nx=5, ny=4, nz=3), {"PTS_3D_VARIANT": "2"}),
+    ("Stokes3D_ve_perf", dict(nx=6, ny=3, nz=4), {"PTS_3D_VARIANT": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=5, ny=4, nz=3), {"PTS_3D_VARIANT": "0"}),
+    ("Stokes3D_ve_multixpu", dict(nx=4, ny=3, nz=5), {"PTS_3D_TWO_KERNEL": "1"}),
+    # padded plane stride; separate (ungrouped) 3D allocations (k3_iter runs)
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_PLANE_PAD": "32"}),
+    ("Stokes3D_ve_multixpu", dict(n=15, rhog_in=-1.0), {"PTS_PLANE_PAD": "544", "PTS_3D_VARIANT": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_3D_GROUPS": "0"}),
 ]
 
 
