@@ -48,7 +48,7 @@ struct pts_ctx {
     int planes_per_block = -160; // 3D single-pass: z planes per block (<0: fewest block rounds, capped)
     int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
-    int variant_3d = 2;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
+    int variant_3d = 1;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
     bool group_3d = true;       // 3D: state fields, partners and τ_o in grouped allocations (ensure_group)
     // adjoint state λ (ping-pong by field id), the pass-1 scratch, the
     // source ∂J/∂Vx, ∂J/∂Vy and the gradient buffer
@@ -115,7 +115,9 @@ int64_t field_guard_elems(const pts_ctx* c) {
     return c->nd == 3 ? (int64_t)c->sxy + 32 * (int64_t)c->sx + 256 : 0;
 }
 
+// FP64 fields only: the FP32 correction state is not read by k3_iter2.
 size_t field_guard_bytes(const pts_ctx* c, size_t elem) {
+    if (elem != sizeof(double)) return 0;
     return ((size_t)field_guard_elems(c) * elem + 255) / 256 * 256;
 }
 

@@ -135,6 +135,7 @@ __global__ __launch_bounds__(64) void k2_dc(const Fields32 in, const Fields32 ou
 // marching in z), viscous, multiply form, with sources.
 // ---------------------------------------------------------------------------
 constexpr int BY3 = 12;
+template <bool E3>  // E3: some extent is 3 (the free-slip copies of put_bc)
 __global__ __launch_bounds__(64 * BY3, 6) void k3_dc(const Fields32 in, const Fields32 out, const StepScalars32 cs,
                                                   int Lz, int ntx, int nty) {
     __shared__ float sh[4][BY3][64];
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(64 * BY3, 6) void k3_dc(const Fields32 in, const Fi
             if (hyz) out.p[PTS_TYZ][oe] = tyz1n;
             if (vz_in && k >= 1) {
                 float R = _dz * (tzz1n - tzz1P) + _dx * (txz1R - txz1n) + _dy * (tyz1D - tyz1n) - _dz * (Pt1n - Pt1P) + sVz;
-                put_bc(out.p[PTS_VZ], o, VzC + R * ((RhP + Rh) * 0.5f), c, nx, 1, d, ny, sx);
+                put_bc<E3>(out.p[PTS_VZ], o, VzC + R * ((RhP + Rh) * 0.5f), c, nx, 1, d, ny, sx);
             }
         }
         const int km = k - 1;
@@ -262,11 +263,11 @@ __global__ __launch_bounds__(64 * BY3, 6) void k3_dc(const Fields32 in, const Fi
             const int q = o - sxy;
             if (vx_in) {
                 float R = _dx * (txx1P - txx1PL) + _dy * (txy1PD - txy1P) + _dz * (txz1n - txz1P) - _dx * (Pt1P - Pt1PL) + sVx;
-                put_bc(out.p[PTS_VX], q, Vx0P + R * ((RhPL + RhP) * 0.5f), d, ny, sx, km, nz, sxy);
+                put_bc<E3>(out.p[PTS_VX], q, Vx0P + R * ((RhPL + RhP) * 0.5f), d, ny, sx, km, nz, sxy);
             }
             if (vy_in) {
                 float R = _dy * (tyy1P - tyy1PU) + _dx * (txy1PR - txy1P) + _dz * (tyz1n - tyz1P) - _dy * (Pt1P - Pt1PU) + sVy;
-                put_bc(out.p[PTS_VY], q, Vy0P + R * ((RhUP + RhP) * 0.5f), c, nx, 1, km, nz, sxy);
+                put_bc<E3>(out.p[PTS_VY], q, Vy0P + R * ((RhUP + RhP) * 0.5f), c, nx, 1, km, nz, sxy);
             }
         }
         Vx0P = Vx0c; Vy0P = Vy0c; VzC = VzN; VzUC = VzUN;
@@ -553,9 +554,12 @@ hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out
         k2_dc<<<nb, 64, 0, s>>>(in, out, c, L, ns);
     } else {
         const int ntx = (in.nx + 1 + 61) / 62, nty = (in.ny + 1 + BY3 - 3) / (BY3 - 2);
-        const int Lz = march_length(in.nz, ntx * nty, device_concurrency(k3_dc, 64 * BY3, conc3), 24);
+        const int Lz = march_length(in.nz, ntx * nty, device_concurrency(k3_dc<false>, 64 * BY3, conc3), 24);
         const int nch = (in.nz + Lz - 1) / Lz;
-        k3_dc<<<ntx * nty * nch, dim3(64, BY3), 0, s>>>(in, out, c, Lz, ntx, nty);
+        if (in.nx == 3 || in.ny == 3 || in.nz == 3)
+            k3_dc<true><<<ntx * nty * nch, dim3(64, BY3), 0, s>>>(in, out, c, Lz, ntx, nty);
+        else
+            k3_dc<false><<<ntx * nty * nch, dim3(64, BY3), 0, s>>>(in, out, c, Lz, ntx, nty);
     }
     return hipGetLastError();
 }

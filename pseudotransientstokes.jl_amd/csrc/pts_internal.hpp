@@ -99,18 +99,28 @@ __device__ __forceinline__ double tau_visc2d_div(double t, double g, double m, d
 // na == 3: the one interior plane feeds both boundary planes), same for b
 // (bc_y!/bc_z! etc., Stokes3D_ve_perf.jl:202-207; edge values end up equal to
 // the diagonal interior neighbour whatever the order of the copies).
-template <typename T>
+// E3 = false drops the extent-3 code (for kernels instantiated for grids
+// with every extent > 3).
+template <bool E3 = true, typename T>
 __device__ __forceinline__ void put_bc(T* V, int q, T v, int a, int na, int sa, int b, int nb, int sb) {
     V[q] = v;
-    const bool alo = a == 1, ahi = a == na - 2, blo = b == 1, bhi = b == nb - 2;
-    if (alo) V[q - sa] = v;
-    if (ahi) V[q + sa] = v;
-    if (blo) V[q - sb] = v;
-    if (bhi) V[q + sb] = v;
-    if (alo && blo) V[q - sa - sb] = v;
-    if (alo && bhi) V[q - sa + sb] = v;
-    if (ahi && blo) V[q + sa - sb] = v;
-    if (ahi && bhi) V[q + sa + sb] = v;
+    const int da = (a == 1) ? -1 : ((a == na - 2) ? 1 : 0);
+    const int db = (b == 1) ? -1 : ((b == nb - 2) ? 1 : 0);
+    if (da) V[q + da * sa] = v;
+    if (db) V[q + db * sb] = v;
+    if (da && db) V[q + da * sa + db * sb] = v;
+    if (E3 && (na == 3 || nb == 3)) {  // grid-uniform: the far-side copies of extent 3
+        const bool a3 = na == 3 && a == 1, b3 = nb == 3 && b == 1;
+        if (a3) {
+            V[q + sa] = v;
+            if (db) V[q + sa + db * sb] = v;
+        }
+        if (b3) {
+            V[q + sb] = v;
+            if (da) V[q + da * sa + sb] = v;
+            if (a3) V[q + sa + sb] = v;
+        }
+    }
 }
 
 // Scalars of one PT iteration.  dp kernels (`dp` launch flag) read Musτ and

@@ -421,16 +421,25 @@ __device__ __forceinline__ void stb(double* p, uint32_t boff, double v) {
 // put_bc with a byte offset (same copies)
 __device__ __forceinline__ void put_bcb(double* V, uint32_t q, double v, int a, int na, int sa, int b, int nb, int sb) {
     stb(V, q, v);
-    const bool alo = a == 1, ahi = a == na - 2, blo = b == 1, bhi = b == nb - 2;
-    const uint32_t ea = (uint32_t)sa * 8u, eb = (uint32_t)sb * 8u;
-    if (alo) stb(V, q - ea, v);
-    if (ahi) stb(V, q + ea, v);
-    if (blo) stb(V, q - eb, v);
-    if (bhi) stb(V, q + eb, v);
-    if (alo && blo) stb(V, q - ea - eb, v);
-    if (alo && bhi) stb(V, q - ea + eb, v);
-    if (ahi && blo) stb(V, q + ea - eb, v);
-    if (ahi && bhi) stb(V, q + ea + eb, v);
+    const int da = (a == 1) ? -1 : ((a == na - 2) ? 1 : 0);
+    const int db = (b == 1) ? -1 : ((b == nb - 2) ? 1 : 0);
+    const uint32_t ea = (uint32_t)(da * sa * 8), eb = (uint32_t)(db * sb * 8);
+    if (da) stb(V, q + ea, v);
+    if (db) stb(V, q + eb, v);
+    if (da && db) stb(V, q + ea + eb, v);
+    if (na == 3 || nb == 3) {  // grid-uniform: the far-side copies of extent 3
+        const bool a3 = na == 3 && a == 1, b3 = nb == 3 && b == 1;
+        const uint32_t fa = (uint32_t)sa * 8u, fb = (uint32_t)sb * 8u;
+        if (a3) {
+            stb(V, q + fa, v);
+            if (db) stb(V, q + fa + eb, v);
+        }
+        if (b3) {
+            stb(V, q + fb, v);
+            if (da) stb(V, q + ea + fb, v);
+            if (a3) stb(V, q + fa + fb, v);
+        }
+    }
 }
 
 template <bool VE, bool G2>
