@@ -32,6 +32,10 @@ __device__ __forceinline__ float fright(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
 }
 
+// The FP32 sweeps keep the arithmetic of the residual that stops them (no
+// FMA contraction, correctly rounded division): with FMA or the 1-ulp
+// hardware reciprocal the correction solve of the 3D random-viscosity 127^3
+// case needs 1.6x the sweeps to reach its relative residual drop.
 __device__ __forceinline__ float tau_visc_f(float t, float g, float m, float e) {
     return (t + 2.0f * g * e) / (1.0f + g / m);
 }

@@ -15,6 +15,7 @@
 // never-updated Dirichlet entries are held by both buffers.
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <type_traits>
 
 #include "pts_internal.hpp"
@@ -464,10 +465,18 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
     // different rows of the planes (rather than eight chunks whose planes lie
     // a multiple of the plane size apart).  Blocks beyond an XCD's share exit.
     const int ntile = ntx * nty;
-    const int xcd = blockIdx.x & 7, seq = blockIdx.x >> 3;
-    const int lo = (int)((int64_t)ntile * xcd / 8), cnt = (int)((int64_t)ntile * (xcd + 1) / 8) - lo;
-    if (seq >= cnt * nchunk) return;
-    const int tile = lo + seq % cnt, chunk = seq / cnt;
+    int tile, chunk;
+    if (nchunk > 0) {
+        const int xcd = blockIdx.x & 7, seq = blockIdx.x >> 3;
+        const int lo = (int)((int64_t)ntile * xcd / 8), cnt = (int)((int64_t)ntile * (xcd + 1) / 8) - lo;
+        if (seq >= cnt * nchunk) return;
+        tile = lo + seq % cnt;
+        chunk = seq / cnt;
+    } else {  // nchunk = 0: contiguous runs of the (chunk, tile) order per XCD
+        const int b = xcd_tile(blockIdx.x, gridDim.x);
+        tile = b % ntile;
+        chunk = b / ntile;
+    }
     const int tx = tile % ntx, ty = tile / ntx;
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -476,9 +485,14 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
     const int nx = f.nx, ny = f.ny, nz = f.nz, sx = f.sx, sxy = f.sxy;
     const int kb = chunk * Lz;
     const int ke = min(kb + Lz, nz);
-    // byte offset of (c, d) within a plane, biased so that c, d >= -1 stay >= 0
+    // Byte offset of the loads of (c, d) within a plane.  Lanes and rows
+    // outside [0, nx] x [0, ny] (halo and edge tiles) load at the nearest
+    // position inside: their values feed no stored result, and the clamp keeps
+    // them on lines that the neighbouring lanes and rows fetch anyway (instead
+    // of the next row's or plane's).  The bias keeps every shifted field offset
+    // non-negative.
     const int bias = sx + 1;
-    const uint32_t tb = (uint32_t)(c + sx * d + bias) * 8u;
+    const uint32_t tb = (uint32_t)(min(max(c, 0), nx) + sx * min(max(d, 0), ny) + bias) * 8u;
 
 
     // element shifts of the fields relative to (c, d, k)
@@ -786,12 +800,23 @@ int concurrency3b() {
 
 template <bool VE, bool MUL, bool BF, bool DP, bool PF, bool G2>
 void go3b(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz) {
+    static const bool map_flat = [] {
+        const char* e = std::getenv("PTS_3D_MAP_FLAT");
+        return e && std::atoi(e) != 0;
+    }();
     constexpr int BY = 12;
     const int ntx = (in.nx + 1 + 61) / 62;
     const int nty = (in.ny + 1 + (BY - 2) - 1) / (BY - 2);
+    const int64_t conc = concurrency3b<VE, MUL, BF, BY, DP, PF, G2>();
+    if (map_flat) {
+        const int L = planes_per_block(Lz, in.nz, (int64_t)ntx * nty, conc);
+        const int nch = (in.nz + L - 1) / L;
+        k3_iter2<VE, MUL, BF, BY, DP, PF, G2><<<(unsigned)(ntx * nty * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, 0);
+        return;
+    }
     // per XCD: ceil(tiles / 8) tiles per chunk on an eighth of the resident blocks
     const int64_t per = ((int64_t)ntx * nty + 7) / 8;
-    const int L = planes_per_block(Lz, in.nz, per, std::max<int64_t>(1, concurrency3b<VE, MUL, BF, BY, DP, PF, G2>() / 8));
+    const int L = planes_per_block(Lz, in.nz, per, std::max<int64_t>(1, conc / 8));
     const int nch = (in.nz + L - 1) / L;
     k3_iter2<VE, MUL, BF, BY, DP, PF, G2><<<(unsigned)(8 * per * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, nch);
 }
