@@ -297,6 +297,7 @@ def main():
         r3 = bench_3d(args.n3d, s3, max(5, args.warmup // 2), ev, args.contexts3d)
         r3["frac_of_peak"] = r3["T_eff_GBs"] / HBM_PEAK_GBS
         r3["workload"] = f"3D Maxwell VE Stokes, nx=ny=nz={args.n3d}, FP64, x-z pure shear"
+        r3["hbm_traffic_bytes"] = load_traffic(f"k3_iter2_{args.n3d}")
         extra["3d_ve"] = r3
         print(f"# 3D VE {args.n3d}^3: {r3['ms_per_it']:.4f} ms/it, T_eff {r3['T_eff_GBs']:.1f} GB/s",
               file=sys.stderr, flush=True)
