@@ -773,10 +773,12 @@ static bool use_dp(const pts_ctx* c);
 
 static int prepare_iter(pts_ctx* c) {
     if (int rc = need_iter_fields(c, true)) return rc;
-    if (c->nd == 2 && use_dp(c) && c->mx_enabled && c->mx_state == 0) {
+    if (c->nd == 3 && c->mx_state == 0 && (c->nx < 3 || c->ny < 3 || c->nz < 3)) c->mx_state = -1;
+    if (use_dp(c) && c->mx_enabled && c->mx_state == 0) {
         if (int rc = ensure(c, PTS_MUSTAU)) return rc;
         HIPCHK(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
-        HIPCHK(c, launch_check_mustau2(c->stream, F(c), c->d_flag));
+        if (c->nd == 2) HIPCHK(c, launch_check_mustau2(c->stream, F(c), c->d_flag));
+        else HIPCHK(c, launch_check_mustau3(c->stream, F(c), c->d_flag));
         int bad = 1;
         HIPCHK(c, hipMemcpyAsync(&bad, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -808,7 +810,7 @@ static void swap_state(pts_ctx* c) {
 // Enqueue n iterations; for the 3D divide form, the last one stores ∇V of
 // the pre-update velocities when store_last_dv is set.
 static bool use_dp(const pts_ctx* c) { return c->dp_ok && c->dp_enabled; }
-static bool use_mx(const pts_ctx* c) { return c->nd == 2 && use_dp(c) && c->mx_enabled && c->mx_state == 1; }
+static bool use_mx(const pts_ctx* c) { return use_dp(c) && c->mx_enabled && c->mx_state == 1; }
 
 static StepScalars with_dp(const pts_ctx* c, StepScalars s) {
     if (use_dp(c)) {
@@ -844,7 +846,7 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s0, int64_t n, bool stor
                                        c->variant_2d, c->persist_2d, dp, use_mx(c)));
             else
                 HIPCHK(c, launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
-                                       c->tile_rows_3d, dp, c->variant_3d, c->slot_bytes));
+                                       c->tile_rows_3d, dp, c->variant_3d, c->slot_bytes, use_mx(c)));
             swap_state(c);
         } else {
             if (store_last_dv && q == n - 1 && !mul) {

@@ -184,6 +184,9 @@ hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, cons
 // 1 in *bad unless Musτ equals compute_maxloc! of μ followed by bc_x!, bc_y!
 // (2D; the precondition of the mx iteration kernels).  *bad must be zero.
 hipError_t launch_check_mustau2(hipStream_t s, const Fields& f, int* bad);
+// 3D: 1 in *bad unless Musτ equals compute_maxloc! of μ followed by bc_x!,
+// bc_y!, bc_z! (the precondition of the mx form of k3_iter2); needs n >= 3.
+hipError_t launch_check_mustau3(hipStream_t s, const Fields& f, int* bad);
 
 // 3D
 hipError_t launch3_compute_P(hipStream_t s, const Fields& f, const StepScalars& c);
@@ -201,7 +204,7 @@ hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 // field_guard_elems in pts_api.hip); otherwise k3_iter runs.
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
                         bool mul, bool bf, int planes_per_block, int tile_rows, bool dp, int variant,
-                        int64_t slot_bytes);
+                        int64_t slot_bytes, bool mx);
 
 // Mixed-precision defect correction (pts_dc.hip)
 hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim);

@@ -398,14 +398,16 @@ struct Fields3g {
 
 struct Plane3 {  // inputs of one plane at (c, d); g/h: Gdτ, dτ_Rho after the exchange
     double g, h, mu, pt, vx, vy, vzn, rg;
+    double mun;  // MX: μ of the next plane
     double tx, txo, ty, tyo, tz, tzo, txy, txyo, txz, txzo, tyz, tyzo;
 };
 struct Nbr3 {  // the same plane from the neighbouring rows
     double gU, rhU, muU, vxU, vzUN, vyN;
+    double muUn, muDn;  // MX: μ of the next plane in the rows above and below
 };
 struct Carry3 {  // plane k-1
-    double vxP, vyP, gP, muP, gLP, muLP, gUP, muUP, rhP, rhUP, rgP;
-    double pt1P, txx1P, tyy1P, tzz1P, txy1P, txz1P, tyz1P, pt1PU, tyy1PU, txy1PD;
+    double vxP, vyP, gP, muP, gUP, muUP, rhP, rhUP, rgP;
+    double pt1P, txx1P, tyy1P, tzz1P, txy1P, txz1P, tyz1P;
     double vzC, vzUC;  // Vz at z-face k (own row, row above)
 };
 
@@ -453,11 +455,15 @@ __device__ __forceinline__ double tau_upd2(double t, double to, double g, double
     }
 }
 
-template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2>
+template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2, bool MX>
 __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const StepScalars cs, int Lz,
                                                     int ntx, int nty, int nchunk) {
-    __shared__ double shA[4][BY][64];  // Pt1, τyy1 (read by row +1); τxy1, τyz1 (read by row -1)
-    __shared__ double shB[7][BY][64];  // next-plane inputs: Gdτ, dτ_Rho, μ, Vx, Vz (row +1), Vy (row -1); Vz (start)
+    // new Pt1, τyy1 (read by row +1) and τxy1, τyz1 (read by row -1), by plane
+    // parity: the V update of plane k-1 re-reads plane k-1's from the other half
+    __shared__ double shA[2][4][BY][64];
+    // next-plane inputs: Gdτ, dτ_Rho, μ (MX: μ of the plane after), Vx, Vz (row +1), Vy (row -1); Vz, μ (start)
+    __shared__ double shB[8][BY][64];
+    static_assert(!(MX && (PF || !DP)), "MX: derived parameters, loads after the arithmetic");
 
     // Block b runs on XCD b % 8.  Each XCD owns a contiguous eighth of the
     // tiles (neighbouring tiles share halo rows in its L2) and takes the
@@ -538,12 +544,25 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
     // Each row role runs its own loop (R = 0: top halo row, 1: own rows,
     // 2: bottom halo row; w is wave-uniform), loading only what it uses.  The
     // loads need no z-range conditions either: the guard zones hold a plane.
-    auto load = [&](auto role, int k, Plane3& I) __attribute__((always_inline)) {
+    // MX (own rows): Musτ of plane k is formed from μ (the cross maximum of
+    // compute_maxloc!, Stokes3D_ve_perf.jl:29-32) except on the boundary
+    // planes and rows and at the first plane of the march, where it is loaded.
+    auto mt_loaded = [&](auto role, int k, bool start) __attribute__((always_inline)) {
+        constexpr int R = decltype(role)::value;
+        if constexpr (!MX || R != 1) return true;
+        else return start || k == 0 || k == nz - 1 || d == 0 || d == ny - 1;
+    };
+    auto load = [&](auto role, int k, Plane3& I, bool start) __attribute__((always_inline)) {
         constexpr int R = decltype(role)::value;
         const uint32_t t = tko(k), ti = t + oin, to = t + oold;
-        I.g = ldb(Gp, t);
+        if (mt_loaded(role, k, start)) I.g = ldb(Gp, t);
         if constexpr (!DP && R != 2) I.h = ldb(Rhp, t);
-        I.mu = ldb(Mup, t);
+        if constexpr (MX) {
+            I.mun = ldb(Mup, t + sxy8);  // μ(k) comes from the previous plane's load
+            if (start) I.mu = ldb(Mup, t);
+        } else {
+            I.mu = ldb(Mup, t);
+        }
         I.vx = ldb(Vxb, ti);
         I.vy = ldb(Vyb, ti);
         I.vzn = ldb(Vzb, ti + sxy8);
@@ -568,42 +587,68 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
             if constexpr (VE) I.tyzo = ldb(tyzb, to);
         }
     };
-    // Gdτ, dτ_Rho of the plane (DP: from Musτ) and the exchange with the
-    // neighbouring rows; `start`: also pass Vz at the first face
-    auto exchange = [&](auto role, Plane3& I, Nbr3& U, bool start, double vzc, double& vzUC)
-                        __attribute__((always_inline)) {
+    // Gdτ, dτ_Rho of plane k (DP: from Musτ) and the exchange with the
+    // neighbouring rows; `start`: also pass Vz (and, MX, μ) of the first plane
+    auto exchange = [&](auto role, int k, Plane3& I, Nbr3& U, const Carry3& P, bool start, double vzc,
+                        double& vzUC) __attribute__((always_inline)) {
         constexpr int R = decltype(role)::value;
         if constexpr (DP) {
+            double M = I.g;
+            if constexpr (MX && R == 1) {
+                if (!mt_loaded(role, k, start)) {
+                    // cross maximum in compute_maxloc!'s order: x, centre, y, z;
+                    // the row neighbours came one plane ahead, μ(k-1) is carried
+                    double m = fmax(from_left(I.mu), from_right(I.mu));
+                    m = fmax(m, I.mu);
+                    m = fmax(m, U.muUn);
+                    m = fmax(m, U.muDn);
+                    m = fmax(m, P.muP);
+                    m = fmax(m, I.mun);
+                    // free-slip copies (bc_x!): the boundary columns take their neighbour's
+                    const double mr = from_right(m), ml = from_left(m);
+                    M = c == 0 ? mr : (c == nx - 1 ? ml : m);
+                }
+            }
             double rh, gd;
-            iter_params_of(I.g, cs, rh, gd);
+            iter_params_of(M, cs, rh, gd);
             I.g = gd;
             I.h = rh;
         }
         if constexpr (R != 2) {  // read by the row below
             shB[0][w][lane] = I.g;
             shB[1][w][lane] = I.h;
-            shB[2][w][lane] = I.mu;
+            if constexpr (!MX) shB[2][w][lane] = I.mu;
             shB[3][w][lane] = I.vx;
             shB[4][w][lane] = I.vzn;
             if (start) shB[6][w][lane] = vzc;
+        }
+        if constexpr (MX) {
+            shB[2][w][lane] = I.mun;  // read by the rows above and below
+            if (start) shB[7][w][lane] = I.mu;
         }
         if constexpr (R != 0) shB[5][w][lane] = I.vy;  // read by the row above
         __syncthreads();
         if constexpr (R != 0) {
             U.gU = shB[0][w - 1][lane];
             U.rhU = shB[1][w - 1][lane];
-            U.muU = shB[2][w - 1][lane];
+            if constexpr (MX) {
+                U.muU = start ? shB[7][w - 1][lane] : U.muUn;
+                U.muUn = shB[2][w - 1][lane];
+            } else {
+                U.muU = shB[2][w - 1][lane];
+            }
             U.vxU = shB[3][w - 1][lane];
             U.vzUN = shB[4][w - 1][lane];
             if (start) vzUC = shB[6][w - 1][lane];
         }
         if constexpr (R != 2) U.vyN = shB[5][w + 1][lane];
+        if constexpr (MX && R == 1) U.muDn = shB[2][w + 1][lane];
     };
 
     auto step = [&](auto role, int k, Plane3& I, Plane3& N, Nbr3& U, Carry3& P) __attribute__((always_inline)) {
         constexpr int R = decltype(role)::value;
         if constexpr (PF) {
-            if (k < k1) load(role, k + 1, N);
+            if (k < k1) load(role, k + 1, N, false);
         }
         const double Gd = I.g, Rh = I.h, Mu = I.mu;
         double Pt1n = 0.0, txx1n = 0.0, tyy1n = 0.0, tzz1n = 0.0, txy1n = 0.0, txz1n = 0.0, tyz1n = 0.0;
@@ -633,8 +678,8 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
             }
             if constexpr (R == 1) {  // τxz at (x-face c, cell d, z-face k)
                 const double VzL = from_left(P.vzC);
-                const double ag = (P.gLP + P.gP + GdL + Gd) * 0.25;
-                const double am = (P.muLP + P.muP + MuL + Mu) * 0.25;
+                const double ag = (from_left(P.gP) + P.gP + GdL + Gd) * 0.25;
+                const double am = (from_left(P.muP) + P.muP + MuL + Mu) * 0.25;
                 const double e = 0.5 * (dfd<MUL>(I.vx - P.vxP, dz, _dz) + dfd<MUL>(P.vzC - VzL, dx, _dx));
                 txz1n = tau_upd2<VE, G2>(I.txz, I.txzo, ag, am, cs, e);
             }
@@ -647,21 +692,18 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
         }
 
         // y-neighbour exchange of the new values
+        const int cb = k & 1, pb = cb ^ 1;
         if constexpr (R != 2) {
-            shA[0][w][lane] = Pt1n;
-            shA[1][w][lane] = tyy1n;
+            shA[cb][0][w][lane] = Pt1n;
+            shA[cb][1][w][lane] = tyy1n;
         }
         if constexpr (R != 0) {
-            shA[2][w][lane] = txy1n;
-            shA[3][w][lane] = tyz1n;
+            shA[cb][2][w][lane] = txy1n;
+            shA[cb][3][w][lane] = tyz1n;
         }
         __syncthreads();
-        double Pt1U = 0.0, tyy1U = 0.0, txy1D = 0.0;
         if constexpr (R == 1) {
-            Pt1U = shA[0][w - 1][lane];   // (c, d-1, k)
-            tyy1U = shA[1][w - 1][lane];
-            txy1D = shA[2][w + 1][lane];  // τxy (c, y-face d+1, k)
-            const double tyz1D = shA[3][w + 1][lane];  // τyz (c, y-face d+1, z-face k)
+            const double tyz1D = shA[cb][3][w + 1][lane];  // τyz (c, y-face d+1, z-face k)
 
             const uint32_t t = tko(k) + oout;
             const double txz1R = from_right(txz1n);
@@ -709,17 +751,20 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
             if (km >= kb && km < ke && km >= 1 && km <= nz - 2) {
                 const double Pt1PL = from_left(P.pt1P), txx1PL = from_left(P.txx1P), RhPL = from_left(P.rhP);
                 const double txy1PR = from_right(P.txy1P);
+                const double Pt1PU = shA[pb][0][w - 1][lane];   // (c, d-1, k-1)
+                const double tyy1PU = shA[pb][1][w - 1][lane];
+                const double txy1PD = shA[pb][2][w + 1][lane];  // τxy (c, y-face d+1, k-1)
                 const uint32_t tq = t - sxy8;
                 if (vx_in) {
-                    const double Rx = dfd<MUL>(P.txx1P - txx1PL, dx, _dx) + dfd<MUL>(P.txy1PD - P.txy1P, dy, _dy) +
+                    const double Rx = dfd<MUL>(P.txx1P - txx1PL, dx, _dx) + dfd<MUL>(txy1PD - P.txy1P, dy, _dy) +
                                       dfd<MUL>(txz1n - P.txz1P, dz, _dz) - dfd<MUL>(P.pt1P - Pt1PL, dx, _dx);
                     put_bcb(Vxm, tq, P.vxP + Rx * ((RhPL + P.rhP) * 0.5), d, ny, sx, km, nz, sxy);
                 } else if (vx_dir) {
                     put_bcb(Vxm, tq, P.vxP, d, ny, sx, km, nz, sxy);
                 }
                 if (vy_in) {
-                    const double Ry = dfd<MUL>(P.tyy1P - P.tyy1PU, dy, _dy) + dfd<MUL>(txy1PR - P.txy1P, dx, _dx) +
-                                      dfd<MUL>(tyz1n - P.tyz1P, dz, _dz) - dfd<MUL>(P.pt1P - P.pt1PU, dy, _dy);
+                    const double Ry = dfd<MUL>(P.tyy1P - tyy1PU, dy, _dy) + dfd<MUL>(txy1PR - P.txy1P, dx, _dx) +
+                                      dfd<MUL>(tyz1n - P.tyz1P, dz, _dz) - dfd<MUL>(P.pt1P - Pt1PU, dy, _dy);
                     put_bcb(Vym, tq, P.vyP + Ry * ((P.rhUP + P.rhP) * 0.5), cl, nx, 1, km, nz, sxy);
                 } else if (vy_dir) {
                     put_bcb(Vym, tq, P.vyP, cl, nx, 1, km, nz, sxy);
@@ -739,8 +784,6 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
         }
         if constexpr (R == 1) {
             P.vxP = I.vx;
-            P.gLP = GdL;
-            P.muLP = MuL;
             P.rhUP = U.rhU;
             P.rhP = Rh;
             if constexpr (BF) P.rgP = I.rg;
@@ -751,15 +794,13 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
             P.txy1P = txy1n;
             P.txz1P = txz1n;
             P.tyz1P = tyz1n;
-            P.pt1PU = Pt1U;
-            P.tyy1PU = tyy1U;
-            P.txy1PD = txy1D;
         }
 
         if (k < k1) {
-            if constexpr (!PF) load(role, k + 1, N);
+            if constexpr (!PF) load(role, k + 1, N, false);
+            if constexpr (MX) N.mu = I.mun;
             double unused = 0.0;
-            exchange(role, N, U, false, 0.0, unused);
+            exchange(role, k + 1, N, U, P, false, 0.0, unused);
         }
     };
 
@@ -767,10 +808,10 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
         Plane3 A{}, B{};
         Nbr3 U{};
         Carry3 P{};
-        load(role, k0, A);
+        load(role, k0, A, true);
         const double vz0 = ldb(Vzb, tko(k0) + oin);  // Vz at z-face k0
         P.vzC = vz0;
-        exchange(role, A, U, true, vz0, P.vzUC);
+        exchange(role, k0, A, U, P, true, vz0, P.vzUC);
         for (int k = k0;;) {
             step(role, k, A, B, U, P);
             if (k == k1) break;
@@ -785,20 +826,20 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
     else run(std::integral_constant<int, 1>{});
 }
 
-template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2>
+template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2, bool MX>
 int concurrency3b() {
     static int cached = 0;
     if (!cached) {
         int dev = 0, cus = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k3_iter2<VE, MUL, BF, BY, DP, PF, G2>, 64 * BY, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k3_iter2<VE, MUL, BF, BY, DP, PF, G2, MX>, 64 * BY, 0);
         cached = std::max(1, cus) * std::max(1, per);
     }
     return cached;
 }
 
-template <bool VE, bool MUL, bool BF, bool DP, bool PF, bool G2>
+template <bool VE, bool MUL, bool BF, bool DP, bool PF, bool G2, bool MX = false>
 void go3b(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz) {
     static const bool map_flat = [] {
         const char* e = std::getenv("PTS_3D_MAP_FLAT");
@@ -807,22 +848,29 @@ void go3b(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz) {
     constexpr int BY = 12;
     const int ntx = (in.nx + 1 + 61) / 62;
     const int nty = (in.ny + 1 + (BY - 2) - 1) / (BY - 2);
-    const int64_t conc = concurrency3b<VE, MUL, BF, BY, DP, PF, G2>();
+    const int64_t conc = concurrency3b<VE, MUL, BF, BY, DP, PF, G2, MX>();
     if (map_flat) {
         const int L = planes_per_block(Lz, in.nz, (int64_t)ntx * nty, conc);
         const int nch = (in.nz + L - 1) / L;
-        k3_iter2<VE, MUL, BF, BY, DP, PF, G2><<<(unsigned)(ntx * nty * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, 0);
+        k3_iter2<VE, MUL, BF, BY, DP, PF, G2, MX><<<(unsigned)(ntx * nty * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, 0);
         return;
     }
     // per XCD: ceil(tiles / 8) tiles per chunk on an eighth of the resident blocks
     const int64_t per = ((int64_t)ntx * nty + 7) / 8;
     const int L = planes_per_block(Lz, in.nz, per, std::max<int64_t>(1, conc / 8));
     const int nch = (in.nz + L - 1) / L;
-    k3_iter2<VE, MUL, BF, BY, DP, PF, G2><<<(unsigned)(8 * per * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, nch);
+    k3_iter2<VE, MUL, BF, BY, DP, PF, G2, MX><<<(unsigned)(8 * per * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, nch);
 }
 
 template <bool VE, bool MUL, bool BF, bool DP>
-void go3b_pf(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bool pf, bool g2) {
+void go3b_pf(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bool pf, bool g2, bool mx) {
+    if constexpr (DP) {
+        if (mx && !pf) {
+            if (g2) go3b<VE, MUL, BF, DP, false, true, true>(s, in, c, Lz);
+            else go3b<VE, MUL, BF, DP, false, false, true>(s, in, c, Lz);
+            return;
+        }
+    }
     if (pf) {
         if (g2) go3b<VE, MUL, BF, DP, true, true>(s, in, c, Lz);
         else go3b<VE, MUL, BF, DP, true, false>(s, in, c, Lz);
@@ -876,7 +924,7 @@ bool group_fields(const Fields& in, const Fields& out, bool ve, bool dp, bool bf
 }  // namespace
 
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve, bool mul,
-                        bool bf, int Lz, int by, bool dp, int variant, int64_t slot_bytes) {
+                        bool bf, int Lz, int by, bool dp, int variant, int64_t slot_bytes, bool mx) {
     if (Lz == 0) Lz = 1;
     Fields3g g;
     if ((variant == 1 || variant == 2) && group_fields(in, out, ve, dp, bf, slot_bytes, g)) {
@@ -887,19 +935,19 @@ hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, cons
         c2.iGdt = g2 ? 1.0 / c.Gdt : 0.0;
         if (ve) {
             if (mul) {
-                if (dp) { if (bf) go3b_pf<true, true, true, true>(s, g, c2, Lz, pf, g2); else go3b_pf<true, true, false, true>(s, g, c2, Lz, pf, g2); }
-                else { if (bf) go3b_pf<true, true, true, false>(s, g, c2, Lz, pf, g2); else go3b_pf<true, true, false, false>(s, g, c2, Lz, pf, g2); }
+                if (dp) { if (bf) go3b_pf<true, true, true, true>(s, g, c2, Lz, pf, g2, mx); else go3b_pf<true, true, false, true>(s, g, c2, Lz, pf, g2, mx); }
+                else { if (bf) go3b_pf<true, true, true, false>(s, g, c2, Lz, pf, g2, mx); else go3b_pf<true, true, false, false>(s, g, c2, Lz, pf, g2, mx); }
             } else {
-                if (dp) { if (bf) go3b_pf<true, false, true, true>(s, g, c2, Lz, pf, g2); else go3b_pf<true, false, false, true>(s, g, c2, Lz, pf, g2); }
-                else { if (bf) go3b_pf<true, false, true, false>(s, g, c2, Lz, pf, g2); else go3b_pf<true, false, false, false>(s, g, c2, Lz, pf, g2); }
+                if (dp) { if (bf) go3b_pf<true, false, true, true>(s, g, c2, Lz, pf, g2, mx); else go3b_pf<true, false, false, true>(s, g, c2, Lz, pf, g2, mx); }
+                else { if (bf) go3b_pf<true, false, true, false>(s, g, c2, Lz, pf, g2, mx); else go3b_pf<true, false, false, false>(s, g, c2, Lz, pf, g2, mx); }
             }
         } else {
             if (mul) {
-                if (dp) { if (bf) go3b_pf<false, true, true, true>(s, g, c2, Lz, pf, false); else go3b_pf<false, true, false, true>(s, g, c2, Lz, pf, false); }
-                else { if (bf) go3b_pf<false, true, true, false>(s, g, c2, Lz, pf, false); else go3b_pf<false, true, false, false>(s, g, c2, Lz, pf, false); }
+                if (dp) { if (bf) go3b_pf<false, true, true, true>(s, g, c2, Lz, pf, false, mx); else go3b_pf<false, true, false, true>(s, g, c2, Lz, pf, false, mx); }
+                else { if (bf) go3b_pf<false, true, true, false>(s, g, c2, Lz, pf, false, mx); else go3b_pf<false, true, false, false>(s, g, c2, Lz, pf, false, mx); }
             } else {
-                if (dp) { if (bf) go3b_pf<false, false, true, true>(s, g, c2, Lz, pf, false); else go3b_pf<false, false, false, true>(s, g, c2, Lz, pf, false); }
-                else { if (bf) go3b_pf<false, false, true, false>(s, g, c2, Lz, pf, false); else go3b_pf<false, false, false, false>(s, g, c2, Lz, pf, false); }
+                if (dp) { if (bf) go3b_pf<false, false, true, true>(s, g, c2, Lz, pf, false, mx); else go3b_pf<false, false, false, true>(s, g, c2, Lz, pf, false, mx); }
+                else { if (bf) go3b_pf<false, false, true, false>(s, g, c2, Lz, pf, false, mx); else go3b_pf<false, false, false, false>(s, g, c2, Lz, pf, false, mx); }
             }
         }
         return hipGetLastError();

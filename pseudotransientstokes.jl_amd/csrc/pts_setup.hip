@@ -153,6 +153,29 @@ __global__ void k_check_mustau2(Fields f, int* bad) {
     }
 }
 
+// 3D: Musτ == maxloc(μ) + bc_x!, bc_y!, bc_z! (Stokes3D_ve_perf.jl:29-32,
+// 178-182), cell by cell and bitwise: after the three copies every cell holds
+// the cross maximum at its position clamped into the interior.
+__global__ void k_check_mustau3(Fields f, int* bad) {
+    const int nx = f.nx, ny = f.ny, nz = f.nz, sx = f.sx, sxy = f.sxy;
+    const int64_t tot = (int64_t)nx * ny * nz;
+    const double* M = f.p[PTS_MUS];
+    const double* Mt = f.p[PTS_MUSTAU];
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((int64_t)nx * ny));
+        const int ic = min(max(i, 1), nx - 2), jc = min(max(j, 1), ny - 2), kc = min(max(k, 1), nz - 2);
+        const int64_t o = ic + sx * (int64_t)jc + (int64_t)sxy * kc;
+        double m = fmax(M[o - 1], M[o + 1]);
+        m = fmax(m, M[o]);
+        m = fmax(m, M[o - sx]);
+        m = fmax(m, M[o + sx]);
+        m = fmax(m, M[o - sxy]);
+        m = fmax(m, M[o + sxy]);
+        const double t = Mt[i + sx * (int64_t)j + (int64_t)sxy * k];
+        if (!(m == t)) atomicOr(bad, 1);
+    }
+}
+
 // bc_x!/bc_y!/bc_z!: copy the first interior plane outward along `dim`.
 __global__ void k_bc(double* a, int64_t n1, int64_t n2, int64_t n3, int sx, int sxy, int dim) {
     int64_t e1 = n1, e2 = n2, e3 = n3;
@@ -302,6 +325,12 @@ hipError_t launch_iter_params(hipStream_t s, const Fields& f, int ndim, bool ve,
 hipError_t launch_check_mustau2(hipStream_t s, const Fields& f, int* bad) {
     const int64_t tot = (int64_t)f.nx * f.ny;
     k_check_mustau2<<<grid_for(tot), kBlock, 0, s>>>(f, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_mustau3(hipStream_t s, const Fields& f, int* bad) {
+    const int64_t tot = (int64_t)f.nx * f.ny * f.nz;
+    k_check_mustau3<<<grid_for(tot), kBlock, 0, s>>>(f, bad);
     return hipGetLastError();
 }
 

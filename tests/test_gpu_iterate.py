@@ -194,6 +194,11 @@ VARIANTS = [
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_PLANE_PAD": "32"}),
     ("Stokes3D_ve_multixpu", dict(n=15, rhog_in=-1.0), {"PTS_PLANE_PAD": "544", "PTS_3D_VARIANT": "1"}),
     ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_3D_GROUPS": "0"}),
+    # Musτ read instead of formed from μ (3D); the formed case runs in the default ones
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_DERIVED_MUSTAU": "0"}),
+    ("Stokes3D_ve_multixpu", dict(n=13, rhog_in=-1.0), {"PTS_DERIVED_MUSTAU": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=3, ny=5, nz=4), {}),
+    ("Stokes3D_ve_perf", dict(nx=125, ny=22, nz=30, ve=False), {"PTS_PLANES_PER_BLOCK": "5"}),
 ]
 
 
