@@ -556,6 +556,7 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
         constexpr int R = decltype(role)::value;
         const uint32_t t = tko(k), ti = t + oin, to = t + oold;
         if (mt_loaded(role, k, start)) I.g = ldb(Gp, t);
+        else if (lane == 0 || lane == 63) I.g = ldb(Gp, t);  // MX: the wave-edge lanes lack an x-neighbour
         if constexpr (!DP && R != 2) I.h = ldb(Rhp, t);
         if constexpr (MX) {
             I.mun = ldb(Mup, t + sxy8);  // μ(k) comes from the previous plane's load
@@ -597,16 +598,19 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
             if constexpr (MX && R == 1) {
                 if (!mt_loaded(role, k, start)) {
                     // cross maximum in compute_maxloc!'s order: x, centre, y, z;
-                    // the row neighbours came one plane ahead, μ(k-1) is carried
+                    // the row neighbours came one plane ahead, μ(k-1) is carried.
+                    // Lanes 0 and 63 (no x-neighbour in the wave) keep the loaded
+                    // Musτ; their Gdτ feeds the edges of lanes 1 and 62.
                     double m = fmax(from_left(I.mu), from_right(I.mu));
                     m = fmax(m, I.mu);
                     m = fmax(m, U.muUn);
                     m = fmax(m, U.muDn);
                     m = fmax(m, P.muP);
                     m = fmax(m, I.mun);
+                    const double m1 = (lane == 0 || lane == 63) ? I.g : m;
                     // free-slip copies (bc_x!): the boundary columns take their neighbour's
-                    const double mr = from_right(m), ml = from_left(m);
-                    M = c == 0 ? mr : (c == nx - 1 ? ml : m);
+                    const double mr = from_right(m1), ml = from_left(m1);
+                    M = c == 0 ? mr : (c == nx - 1 ? ml : m1);
                 }
             }
             double rh, gd;

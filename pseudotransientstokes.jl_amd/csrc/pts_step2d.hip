@@ -233,9 +233,12 @@ __device__ __forceinline__ void load_row2(Row2& w, const Fields& in, const StepS
 // exactly the row it needs.  Lanes outside the grid then hold copies of edge
 // values; no stored output depends on them (each output's operands come from
 // lanes and rows inside its own validity range).
+// MX == 2: lanes with `mxload` (boundary columns and the strip's halo lanes)
+// fetch Musτ itself with the row, into Gd (it equals the cross maximum: the
+// MX kernels only run after a device check of that).
 template <bool VE, bool BF, bool DP, int MX>
 __device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, const StepScalars& c, int cx, int J,
-                                                  int nx, int sx) {
+                                                  int nx, int sx, bool mxload = false) {
     const int oc = min(max(cx, 0), nx - 1) + sx * J;
     const int of = min(max(cx, 0), nx) + sx * J;
     const int ov = min(max(cx - 1, 0), nx - 2) + sx * max(J - 1, 0);
@@ -250,6 +253,7 @@ __device__ __forceinline__ void load_row2_clamped(Row2& w, const Fields& in, con
     }
     if constexpr (DP && MX == 2) {
         w.Gd = w.Rh = 0.0;  // derived from μ in the row step
+        if (mxload) w.Gd = in.p[PTS_MUSTAU][oc];
     } else if constexpr (DP) {
         iter_params_of(MX ? mustau_of(in, cx, J) : in.p[PTS_MUSTAU][oc], c, w.Rh, w.Gd);
     } else {
@@ -295,7 +299,8 @@ __device__ __forceinline__ void st(double* p, double v) {
 // pipeline: x-neighbours by DPP (wave edges: exchanged one row ahead through
 // LDS), the row below from registers, the row above from the prefetched row;
 // rows whose cross is not in the pipeline (first and last row of a chunk,
-// grid row 0) and the boundary columns fall back to the loads.
+// grid row 0), the boundary columns and the strip's two halo lanes fall
+// back to the loads.
 template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, int MX>
 __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
                                                    int nstrips, int ntiles) {
@@ -328,6 +333,7 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
         const bool fin = cx >= 0 && cx <= nx;      // Vx face
         const bool vin = cx >= 1 && cx <= nx - 1;  // vertex column cx-1 / interior Vx face
         const bool vyin = cx >= 1 && cx <= nx - 2; // interior Vy column
+        const bool mxload = cx == 0 || cx == nx - 1 || gl == 0 || gl == 64 * NW - 1;
 
         // values carried from row J-1
         double VxP = 0.0, GdP = 0.0, MuP = 0.0, GdPl = 0.0, MuPl = 0.0, RhP = 0.0, rgP = 0.0;
@@ -350,7 +356,10 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
                     Mt = MtP;  // Musτ(:, ny-1) = Musτ(:, ny-2)
                 } else {
                     Mt = fmax(fmax(fmax(fmax(muL, muR), cur.Mu), MuP), nxtMu);
-                    if (cx == 0 || cx == nx - 1) Mt = mustau_of(in, cx, J);  // clamped column
+                    // clamped boundary columns, and the strip's two halo lanes, which
+                    // have no x-neighbour in the block (their Gdτ feeds the
+                    // vertices of the first and last own lanes): Musτ as loaded
+                    if (mxload) Mt = cur.Gd;
                 }
                 MtP = Mt;
                 iter_params_of(Mt, c, cur.Rh, cur.Gd);
@@ -479,12 +488,12 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
         };
         Row2 ra, rb;
         if constexpr (PF) {
-            load_row2_clamped<VE, BF, DP, MX>(ra, in, c, cx, j0, nx, sx);
+            load_row2_clamped<VE, BF, DP, MX>(ra, in, c, cx, j0, nx, sx, mxload);
             for (int J = j0; J <= j1; J += 2) {
-                if (J + 1 <= j1) load_row2_clamped<VE, BF, DP, MX>(rb, in, c, cx, J + 1, nx, sx);
+                if (J + 1 <= j1) load_row2_clamped<VE, BF, DP, MX>(rb, in, c, cx, J + 1, nx, sx, mxload);
                 step(J, ra, J + 1 <= j1 ? rb.Mu : ra.Mu);
                 if (J + 1 > j1) break;
-                if (J + 2 <= j1) load_row2_clamped<VE, BF, DP, MX>(ra, in, c, cx, J + 2, nx, sx);
+                if (J + 2 <= j1) load_row2_clamped<VE, BF, DP, MX>(ra, in, c, cx, J + 2, nx, sx, mxload);
                 step(J + 1, rb, J + 2 <= j1 ? ra.Mu : rb.Mu);
             }
         } else {

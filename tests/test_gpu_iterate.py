@@ -134,6 +134,37 @@ def test_cached_graphs_follow_new_scalars():
     b.close()
 
 
+@pytest.mark.parametrize("preset,kw", [("Stokes3D_ve_perf", dict(nx=140, ny=33, nz=20)),
+                                       ("Stokes3D_ve_multixpu", dict(nx=125, ny=21, nz=17, rhog_in=-1.0)),
+                                       ("Stokes3D_ve_perf", dict(nx=63, ny=41, nz=14, ve=False)),
+                                       ("Stokes2D_ve_perf", dict(nx=1100, ny=75)),
+                                       ("Stokes2D", dict(nx=600, ny=130, rhog_in=-1.0)),
+                                       ("Stokes2D_ve", dict(nx=509, ny=67, dt=0.1))])
+def test_formed_mustau_rough_viscosity(preset, kw):
+    # A rough μ (every cross maximum may sit at any neighbour, also across the
+    # 62-column tile edges) with Musτ = maxloc(μ) + the free-slip copies, so
+    # that the kernels form Musτ from μ; the oracle reads the same Musτ, Gdτ
+    # and dτ_Rho.
+    m, o = pair(preset, **kw)
+    rng = np.random.default_rng(11)
+    mu = np.exp(rng.normal(0.0, 1.5, size=o.shape("Mus")))
+    o.set("Mus", mu)
+    m.ctx.upload("Mus", mu)
+    m.ctx.maxloc("Mustau", "Mus")
+    for bc in (m.ctx.bc_x, m.ctx.bc_y, m.ctx.bc_z)[:m.p["ndim"]]:
+        bc("Mustau")
+    p = m.p
+    m.ctx.iter_params(m.Vpdtau, p["G"], p["dt"], p["Re"], p["r"], m.max_l)
+    for f in ("Mustau", "Gdtau", "dtau_Rho"):
+        o.set(f, m.get(f))
+    randomise(m, o, seed=7)
+    for n in (1, 2, 5):
+        m.iterate(n)
+        o.iterate(n)
+        assert compare(m, o, STATE) == [], n
+    m.close()
+
+
 VARIANTS = [
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_3D_TWO_KERNEL": "1"}),
     ("Stokes3D_ve_multixpu", dict(n=21), {"PTS_3D_TWO_KERNEL": "1"}),
