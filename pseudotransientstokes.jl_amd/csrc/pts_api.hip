@@ -50,6 +50,7 @@ struct pts_ctx {
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
     int variant_3d = 1;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
     bool group_3d = true;       // 3D: state fields, partners and τ_o in grouped allocations (ensure_group)
+    bool map_flat_3d = false;   // k3_iter2: tiles in the flat (chunk, tile) order
     // adjoint state λ (ping-pong by field id), the pass-1 scratch, the
     // source ∂J/∂Vx, ∂J/∂Vy and the gradient buffer
     double* adj[2][PTS_NFIELDS] = {};
@@ -341,6 +342,7 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_VARIANT")) c->variant_3d = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_GROUPS")) c->group_3d = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_3D_MAP_FLAT")) c->map_flat_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_DERIVED_PARAMS")) c->dp_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_DERIVED_MUSTAU")) c->mx_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_ADJ_TWO_PASS")) c->adj_fused = std::atoi(e) == 0;
@@ -846,7 +848,8 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s0, int64_t n, bool stor
                                        c->variant_2d, c->persist_2d, dp, use_mx(c)));
             else
                 HIPCHK(c, launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
-                                       c->tile_rows_3d, dp, c->variant_3d, c->slot_bytes, use_mx(c)));
+                                       c->tile_rows_3d, dp, c->variant_3d, c->slot_bytes, use_mx(c),
+                                       c->map_flat_3d));
             swap_state(c);
         } else {
             if (store_last_dv && q == n - 1 && !mul) {

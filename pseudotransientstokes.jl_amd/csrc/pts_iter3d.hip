@@ -15,7 +15,6 @@
 // never-updated Dirichlet entries are held by both buffers.
 #include <cmath>
 #include <cstdint>
-#include <cstdlib>
 #include <type_traits>
 
 #include "pts_internal.hpp"
@@ -844,11 +843,7 @@ int concurrency3b() {
 }
 
 template <bool VE, bool MUL, bool BF, bool DP, bool PF, bool G2, bool MX = false>
-void go3b(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz) {
-    static const bool map_flat = [] {
-        const char* e = std::getenv("PTS_3D_MAP_FLAT");
-        return e && std::atoi(e) != 0;
-    }();
+void go3b(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bool map_flat) {
     constexpr int BY = 12;
     const int ntx = (in.nx + 1 + 61) / 62;
     const int nty = (in.ny + 1 + (BY - 2) - 1) / (BY - 2);
@@ -867,20 +862,21 @@ void go3b(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz) {
 }
 
 template <bool VE, bool MUL, bool BF, bool DP>
-void go3b_pf(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bool pf, bool g2, bool mx) {
+void go3b_pf(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bool pf, bool g2, bool mx,
+             bool map_flat) {
     if constexpr (DP) {
         if (mx && !pf) {
-            if (g2) go3b<VE, MUL, BF, DP, false, true, true>(s, in, c, Lz);
-            else go3b<VE, MUL, BF, DP, false, false, true>(s, in, c, Lz);
+            if (g2) go3b<VE, MUL, BF, DP, false, true, true>(s, in, c, Lz, map_flat);
+            else go3b<VE, MUL, BF, DP, false, false, true>(s, in, c, Lz, map_flat);
             return;
         }
     }
     if (pf) {
-        if (g2) go3b<VE, MUL, BF, DP, true, true>(s, in, c, Lz);
-        else go3b<VE, MUL, BF, DP, true, false>(s, in, c, Lz);
+        if (g2) go3b<VE, MUL, BF, DP, true, true>(s, in, c, Lz, map_flat);
+        else go3b<VE, MUL, BF, DP, true, false>(s, in, c, Lz, map_flat);
     } else {
-        if (g2) go3b<VE, MUL, BF, DP, false, true>(s, in, c, Lz);
-        else go3b<VE, MUL, BF, DP, false, false>(s, in, c, Lz);
+        if (g2) go3b<VE, MUL, BF, DP, false, true>(s, in, c, Lz, map_flat);
+        else go3b<VE, MUL, BF, DP, false, false>(s, in, c, Lz, map_flat);
     }
 }
 
@@ -928,7 +924,8 @@ bool group_fields(const Fields& in, const Fields& out, bool ve, bool dp, bool bf
 }  // namespace
 
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve, bool mul,
-                        bool bf, int Lz, int by, bool dp, int variant, int64_t slot_bytes, bool mx) {
+                        bool bf, int Lz, int by, bool dp, int variant, int64_t slot_bytes, bool mx,
+                        bool map_flat) {
     if (Lz == 0) Lz = 1;
     Fields3g g;
     if ((variant == 1 || variant == 2) && group_fields(in, out, ve, dp, bf, slot_bytes, g)) {
@@ -939,19 +936,19 @@ hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, cons
         c2.iGdt = g2 ? 1.0 / c.Gdt : 0.0;
         if (ve) {
             if (mul) {
-                if (dp) { if (bf) go3b_pf<true, true, true, true>(s, g, c2, Lz, pf, g2, mx); else go3b_pf<true, true, false, true>(s, g, c2, Lz, pf, g2, mx); }
-                else { if (bf) go3b_pf<true, true, true, false>(s, g, c2, Lz, pf, g2, mx); else go3b_pf<true, true, false, false>(s, g, c2, Lz, pf, g2, mx); }
+                if (dp) { if (bf) go3b_pf<true, true, true, true>(s, g, c2, Lz, pf, g2, mx, map_flat); else go3b_pf<true, true, false, true>(s, g, c2, Lz, pf, g2, mx, map_flat); }
+                else { if (bf) go3b_pf<true, true, true, false>(s, g, c2, Lz, pf, g2, mx, map_flat); else go3b_pf<true, true, false, false>(s, g, c2, Lz, pf, g2, mx, map_flat); }
             } else {
-                if (dp) { if (bf) go3b_pf<true, false, true, true>(s, g, c2, Lz, pf, g2, mx); else go3b_pf<true, false, false, true>(s, g, c2, Lz, pf, g2, mx); }
-                else { if (bf) go3b_pf<true, false, true, false>(s, g, c2, Lz, pf, g2, mx); else go3b_pf<true, false, false, false>(s, g, c2, Lz, pf, g2, mx); }
+                if (dp) { if (bf) go3b_pf<true, false, true, true>(s, g, c2, Lz, pf, g2, mx, map_flat); else go3b_pf<true, false, false, true>(s, g, c2, Lz, pf, g2, mx, map_flat); }
+                else { if (bf) go3b_pf<true, false, true, false>(s, g, c2, Lz, pf, g2, mx, map_flat); else go3b_pf<true, false, false, false>(s, g, c2, Lz, pf, g2, mx, map_flat); }
             }
         } else {
             if (mul) {
-                if (dp) { if (bf) go3b_pf<false, true, true, true>(s, g, c2, Lz, pf, false, mx); else go3b_pf<false, true, false, true>(s, g, c2, Lz, pf, false, mx); }
-                else { if (bf) go3b_pf<false, true, true, false>(s, g, c2, Lz, pf, false, mx); else go3b_pf<false, true, false, false>(s, g, c2, Lz, pf, false, mx); }
+                if (dp) { if (bf) go3b_pf<false, true, true, true>(s, g, c2, Lz, pf, false, mx, map_flat); else go3b_pf<false, true, false, true>(s, g, c2, Lz, pf, false, mx, map_flat); }
+                else { if (bf) go3b_pf<false, true, true, false>(s, g, c2, Lz, pf, false, mx, map_flat); else go3b_pf<false, true, false, false>(s, g, c2, Lz, pf, false, mx, map_flat); }
             } else {
-                if (dp) { if (bf) go3b_pf<false, false, true, true>(s, g, c2, Lz, pf, false, mx); else go3b_pf<false, false, false, true>(s, g, c2, Lz, pf, false, mx); }
-                else { if (bf) go3b_pf<false, false, true, false>(s, g, c2, Lz, pf, false, mx); else go3b_pf<false, false, false, false>(s, g, c2, Lz, pf, false, mx); }
+                if (dp) { if (bf) go3b_pf<false, false, true, true>(s, g, c2, Lz, pf, false, mx, map_flat); else go3b_pf<false, false, false, true>(s, g, c2, Lz, pf, false, mx, map_flat); }
+                else { if (bf) go3b_pf<false, false, true, false>(s, g, c2, Lz, pf, false, mx, map_flat); else go3b_pf<false, false, false, false>(s, g, c2, Lz, pf, false, mx, map_flat); }
             }
         }
         return hipGetLastError();

@@ -229,6 +229,7 @@ VARIANTS = [
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12), {"PTS_DERIVED_MUSTAU": "0"}),
     ("Stokes3D_ve_multixpu", dict(n=13, rhog_in=-1.0), {"PTS_DERIVED_MUSTAU": "0"}),
     ("Stokes3D_ve_perf", dict(nx=3, ny=5, nz=4), {}),
+    ("Stokes3D_ve_perf", dict(nx=140, ny=30, nz=45), {"PTS_3D_MAP_FLAT": "1"}),
     ("Stokes3D_ve_perf", dict(nx=125, ny=22, nz=30, ve=False), {"PTS_PLANES_PER_BLOCK": "5"}),
 ]
 
