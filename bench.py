@@ -277,7 +277,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--n-tts", type=int, default=127, help="time-to-tolerance grid (0: skip)")
-    ap.add_argument("--cpu-3d", action="store_true", help="also time the oracle on the 3D block")
+    ap.add_argument("--no-cpu-3d", action="store_true", help="skip the oracle's time on the 3D block")
     ap.add_argument("--no-extra", action="store_true", help="skip the 127^2 and 2047^2 extra lines")
     ap.add_argument("--tts-no-graph", action="store_true",
                     help="FP64 time-to-tolerance without HIP graphs (for tracing tools)")
@@ -317,9 +317,9 @@ def main():
             print(f"# 127^2 viscous: GPU {c1['gpu_s']} s, CPU {c1['cpu_s']} s ({c1['gpu_iters']} iterations)",
                   file=sys.stderr, flush=True)
     cpu = None
-    if not args.no_cpu and not args.no_3d and args.cpu_3d:
+    if not args.no_cpu and not args.no_3d and not args.no_cpu_3d:
         threads = min(16, os.cpu_count() or 1)
-        c3 = cpu_baseline_3d(args.n3d, 1, threads)
+        c3 = cpu_baseline_3d(args.n3d, 2, threads)
         extra["3d_ve"]["cpu_baseline"] = c3
         extra["3d_ve"]["gpu_over_cpu"] = round(extra["3d_ve"]["T_eff_GBs"] / c3["value"], 1)
         print(f"# CPU 3D: {c3['value']:.2f} GB/s ({c3['sample']})", file=sys.stderr, flush=True)
