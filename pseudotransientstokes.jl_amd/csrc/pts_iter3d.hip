@@ -13,6 +13,11 @@
 // plane), y-neighbours of read-only inputs from the cache, z-neighbours from
 // registers.  State is read from `in` and written to `out` (ping-pong); the
 // never-updated Dirichlet entries are held by both buffers.
+//
+// Two kernels share this structure: k3_iter (predicated loads, any layout)
+// and k3_iter2 (the default: grouped allocations, no load predicates, LDS
+// exchange of the row-neighbour inputs, Musτ formed from μ); launch3_iter at
+// the end of the file picks one.
 #include <cmath>
 #include <cstdint>
 #include <type_traits>
@@ -364,19 +369,24 @@ void go3(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& 
 
 // ---------------------------------------------------------------------------
 // k3_iter2 -- the same iteration and traversal, with less work per cell:
-//   * one 32-bit byte offset per thread addresses every load and store: each
-//     field's pointer is moved to the current plane and carries the field's
-//     fixed index shift (txx at (c, d-1, k-1), edges at (c-1, d-1, k-1), ...),
-//     so the loads use the scalar-base form and need no per-array address;
-//   * loads carry no lane predicates: 3D field allocations have guard zones
-//     around the strided box that absorb the halo lanes and rows beyond the
-//     extents; only the z-ranges (uniform per plane) gate loads.  Values read
-//     outside a field's extent only feed lanes whose results are not stored;
+//   * one 32-bit byte offset per thread and plane addresses every load and
+//     store: each field's pointer carries the field's fixed index shift (txx
+//     at (c, d-1, k-1), edges at (c-1, d-1, k-1), ...) and the input, output
+//     and old-stress buffers are slots of one allocation, so the loads use the
+//     scalar-base form with 12 base pointers;
+//   * loads carry no lane or z-range predicates: FP64 3D allocations have
+//     guard zones (a plane and 32 rows) around the strided box, and lanes or
+//     rows outside [0, nx] x [0, ny] load at the nearest position inside.
+//     Such values only feed results that are not stored;
 //   * the row-above inputs (Gdτ, dτ_Rho, μ, Vx, Vz) and the row-below Vy come
 //     from LDS, written by the neighbouring row one plane ahead, instead of
 //     being loaded again and (Gdτ, dτ_Rho) recomputed from Musτ;
-//   * the halo rows run only what their neighbour row needs (the top row its
-//     Pt and τyy, the bottom row its τxy and τyz), on wave-uniform branches;
+//   * each row role runs its own loop: the top halo row only forms Pt and τyy
+//     for its neighbour, the bottom halo row τxy and τyz;
+//   * the tiles are dealt so that each XCD sweeps its own eighth of the tiles
+//     chunk by chunk, and the march length minimises the rounds of blocks;
+//   * MX: the own rows form Musτ from μ (the cross maximum of
+//     compute_maxloc!), one array fewer per iteration;
 //   * PF: the loads of plane k+1 are issued before the arithmetic of plane k;
 //   * G2: with G·dt a power of two, τ·Gr uses Gdτ·(1/(G·dt)), which rounds
 //     exactly as Gdτ/(G·dt).
