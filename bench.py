@@ -156,6 +156,43 @@ def bench_tts(n, nout=500, rtol_inner=1e-4, use_graph=True):
                 dc_vs_fp64_max_rel_err=rel)
 
 
+def bench_c3_solve(n=511, nt=5, chunk=10000):
+    """Time to solution of the 3D Maxwell VE block (Stokes3D_ve_perf.jl
+    kernels, G = 1, dt = 1, x-z pure shear, 1e-3 sphere): nt physical steps,
+    each PT loop run to ‖res‖∞ < 1e-8 with a check every 200 iterations.  The
+    loop is driven in chunks of `chunk` iterations (a multiple of the check
+    interval, so the checks and the result are those of one loop) to report
+    progress."""
+    from ptstokes import StokesModel
+    m = StokesModel("Stokes3D_ve_perf", n=n, nt=nt, iter_max=10 ** 7, nout=200, norm_kind=1, eps=1e-8).setup()
+    p = m.p
+    m.ctx.sync()
+    steps = []
+    t_all = time.perf_counter()
+    for it in range(nt):
+        m.ctx.assign_tau()
+        t0 = time.perf_counter()
+        iters = 0
+        while True:
+            r = m.ctx.pt_solve(m.pt_params(), p["eps"], p["nout"], chunk - 1, p["norm_kind"], use_graph=True)
+            iters += r["iters"]
+            print(f"# 3D VE {n}^3 step {it + 1}: {iters} iterations, err {r['err']:.3e}", file=sys.stderr,
+                  flush=True)
+            if r["converged"]:
+                break
+            if iters >= 40 * chunk:
+                raise RuntimeError(f"3D VE {n}^3 step {it + 1} did not converge in {iters} iterations")
+        steps.append(dict(iters=iters, s=round(time.perf_counter() - t0, 2), err=r["err"]))
+    wall = time.perf_counter() - t_all
+    total = sum(st["iters"] for st in steps)
+    m.close()
+    a_eff = 29.0 * n ** 3 * 8
+    return dict(workload=f"3D Maxwell VE Stokes, nx=ny=nz={n}, FP64, {nt} physical steps, each to "
+                         "||res||_inf < 1e-8 (check every 200 iterations)",
+                steps=steps, iterations=total, wall_s=round(wall, 2), ms_per_it=round(wall * 1e3 / total, 4),
+                T_eff_GBs=round(a_eff * total / wall / 1e9, 1))
+
+
 def bench_c1(n=127, threads=16):
     """2D viscous Stokes (Stokes2D.jl, divide form) at 127²: 1e-3 inclusion of
     radius 1 with ρg_y = -1 inside, free-slip walls (εbg = 0), Re and r at the
@@ -279,6 +316,8 @@ def main():
     ap.add_argument("--n-tts", type=int, default=127, help="time-to-tolerance grid (0: skip)")
     ap.add_argument("--no-cpu-3d", action="store_true", help="skip the oracle's time on the 3D block")
     ap.add_argument("--no-extra", action="store_true", help="skip the 127^2 and 2047^2 extra lines")
+    ap.add_argument("--c3-solve-steps", type=int, default=0,
+                    help="also run that many converged physical steps of the 3D VE block (minutes)")
     ap.add_argument("--tts-no-graph", action="store_true",
                     help="FP64 time-to-tolerance without HIP graphs (for tracing tools)")
     args = ap.parse_args()
@@ -316,6 +355,11 @@ def main():
             extra["c1_viscous_127_time_to_solution"] = c1
             print(f"# 127^2 viscous: GPU {c1['gpu_s']} s, CPU {c1['cpu_s']} s ({c1['gpu_iters']} iterations)",
                   file=sys.stderr, flush=True)
+    if args.c3_solve_steps > 0:
+        c3s = bench_c3_solve(args.n3d, args.c3_solve_steps)
+        extra["c3_3d_ve_time_to_solution"] = c3s
+        print(f"# 3D VE {args.n3d}^3 time to solution: {c3s['iterations']} iterations in {c3s['wall_s']} s",
+              file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu and not args.no_3d and not args.no_cpu_3d:
         threads = min(16, os.cpu_count() or 1)
