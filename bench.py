@@ -95,14 +95,21 @@ def bench_3d(n, steps, warmup, ev, contexts=1):
     kernel's HBM rate depends on where the 28 arrays land physically (which
     channels concurrently streamed arrays share), which differs from one
     allocation to the next, so the median over several is reported with all
-    of them."""
+    of them.  The library selects among a few placements on a context's first
+    iteration call (pts_placement_times): that call is timed on its own and
+    its kernel times are listed per context."""
     from ptstokes import StokesModel
-    per_ctx, walls = [], []
+    per_ctx, walls, place_ms, place_s = [], [], [], []
     a_eff = 29.0 * n ** 3 * 8
     dev_bytes = 0
     for _ in range(max(1, contexts)):
         m = StokesModel("Stokes3D_ve_perf", n=n).setup()
         m.ctx.assign_tau()
+        t0 = time.perf_counter()
+        m.ctx.pt_iterate(m.pt_params(), 0)  # first call: placement of the field blocks, no iteration
+        m.ctx.sync()
+        place_s.append(round(time.perf_counter() - t0, 3))
+        place_ms.append(m.ctx.placement_times())
         ms, wall = time_iterations(m, steps, warmup, ev)
         per_ctx.append(ms / steps)
         walls.append(wall * 1e3 / steps)
@@ -113,7 +120,8 @@ def bench_3d(n, steps, warmup, ev, contexts=1):
     t_it = per_ctx[mid] / 1e3
     out = dict(n=n, steps=steps, ms_per_it=per_ctx[mid], wall_ms_per_it=walls[mid], A_eff_bytes=a_eff,
                T_eff_GBs=a_eff / t_it / 1e9, device_bytes=dev_bytes,
-               ms_per_it_by_context=[round(t, 4) for t in per_ctx])
+               ms_per_it_by_context=[round(t, 4) for t in per_ctx], placement_ms_by_context=place_ms,
+               placement_s_by_context=place_s)
     return out
 
 
@@ -338,6 +346,7 @@ def main():
         r3["workload"] = f"3D Maxwell VE Stokes, nx=ny=nz={args.n3d}, FP64, x-z pure shear"
         r3["hbm_traffic_bytes"] = load_traffic(f"k3_iter2_{args.n3d}")
         extra["3d_ve"] = r3
+        print(f"# 3D VE {args.n3d}^3 placements tried (ms): {r3['placement_ms_by_context']}", file=sys.stderr)
         print(f"# 3D VE {args.n3d}^3: {r3['ms_per_it']:.4f} ms/it, T_eff {r3['T_eff_GBs']:.1f} GB/s",
               file=sys.stderr, flush=True)
     if args.n_tts > 0:

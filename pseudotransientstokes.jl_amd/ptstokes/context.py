@@ -104,6 +104,12 @@ class Context:
         self._chk(self.lib.pts_device_bytes(self.h, ctypes.byref(b)))
         return b.value
 
+    def placement_times(self):
+        """Kernel times (ms) of the 3D block placements tried: [original, try 1, ...]."""
+        ms = (ctypes.c_float * 8)()
+        self._chk(self.lib.pts_placement_times(self.h, ms))
+        return [round(float(x), 4) for x in ms if x > 0]
+
     # -- setup kernels -------------------------------------------------------
     def init_inclusion(self, lx, ly, lz=10.0, mus0=1.0, musi=1e-3, rad2=1.0, ebg=1.0, rhog_in=0.0,
                        rhog_out=0.0, igg=False):

@@ -4,7 +4,7 @@ exercise strip and chunk edges."""
 import numpy as np
 import pytest
 
-from gpu_cases import STATE, compare, pair, randomise
+from gpu_cases import HIST, STATE, compare, pair, randomise
 
 pytestmark = pytest.mark.gpu
 
@@ -134,6 +134,36 @@ def test_cached_graphs_follow_new_scalars():
     b.close()
 
 
+@pytest.mark.parametrize("accept", ["1", "0"])
+def test_moved_blocks_keep_state_and_graphs(accept, monkeypatch):
+    # a context whose field blocks are moved (or tried and moved back) before
+    # its first solve iterates exactly like one that keeps its blocks, through
+    # captured graphs, the per-call path and later downloads
+    monkeypatch.setenv("PTS_3D_PLACE_MIN_MB", "0")
+    monkeypatch.setenv("PTS_3D_PLACE_ACCEPT", accept)
+    monkeypatch.setenv("PTS_3D_PLACE_TRIES", "3")
+    a, _ = pair("Stokes3D_ve_perf", nx=45, ny=26, nz=21)
+    monkeypatch.setenv("PTS_3D_PLACE_TRIES", "0")
+    b, _ = pair("Stokes3D_ve_perf", nx=45, ny=26, nz=21)
+    nbytes = a.ctx.device_bytes()
+    params = a.pt_params()
+    a.ctx.pt_solve(params, 1e-300, 10, 29, 0, use_graph=True)  # 30 iterations, no convergence
+    b.ctx.pt_iterate(params, 30)
+    assert len(a.ctx.placement_times()) == 4 and b.ctx.placement_times() == []
+    assert a.ctx.device_bytes() == nbytes
+    for f in STATE:
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.ctx.assign_tau()
+    b.ctx.assign_tau()
+    a.ctx.pt_solve(params, 1e-300, 10, 19, 0, use_graph=True)
+    a.ctx.pt_iterate(params, 3)
+    b.ctx.pt_iterate(params, 23)
+    for f in STATE + HIST + ["Mus"]:
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("preset,kw", [("Stokes3D_ve_perf", dict(nx=140, ny=33, nz=20)),
                                        ("Stokes3D_ve_multixpu", dict(nx=125, ny=21, nz=17, rhog_in=-1.0)),
                                        ("Stokes3D_ve_perf", dict(nx=63, ny=41, nz=14, ve=False)),
@@ -231,6 +261,12 @@ VARIANTS = [
     ("Stokes3D_ve_perf", dict(nx=3, ny=5, nz=4), {}),
     ("Stokes3D_ve_perf", dict(nx=140, ny=30, nz=45), {"PTS_3D_MAP_FLAT": "1"}),
     ("Stokes3D_ve_perf", dict(nx=125, ny=22, nz=30, ve=False), {"PTS_PLANES_PER_BLOCK": "5"}),
+    # placement selection moved into every size: blocks moved on every try,
+    # never, or when faster
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12),
+     {"PTS_3D_PLACE_MIN_MB": "0", "PTS_3D_PLACE_ACCEPT": "1", "PTS_3D_PLACE_TRIES": "2"}),
+    ("Stokes3D_ve_multixpu", dict(n=15, rhog_in=-1.0), {"PTS_3D_PLACE_MIN_MB": "0", "PTS_3D_PLACE_ACCEPT": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_3D_PLACE_MIN_MB": "0"}),
 ]
 
 
