@@ -164,15 +164,13 @@ def bench_tts(n, nout=500, rtol_inner=1e-4, use_graph=True):
                 dc_vs_fp64_max_rel_err=rel)
 
 
-def bench_c3_solve(n=511, nt=5, chunk=10000):
-    """Time to solution of the 3D Maxwell VE block (Stokes3D_ve_perf.jl
-    kernels, G = 1, dt = 1, x-z pure shear, 1e-3 sphere): nt physical steps,
-    each PT loop run to ‖res‖∞ < 1e-8 with a check every 200 iterations.  The
-    loop is driven in chunks of `chunk` iterations (a multiple of the check
-    interval, so the checks and the result are those of one loop) to report
-    progress."""
+def bench_solve(preset, n, nt, a_eff, label, chunk=10000, **kw):
+    """Time to solution of nt physical steps of a preset (assign τ_o, then the
+    PT loop to ‖res‖∞ < 1e-8 checked every 200 iterations).  The loop is
+    driven in chunks of `chunk` iterations (a multiple of the check interval,
+    so the checks and the result are those of one loop) to report progress."""
     from ptstokes import StokesModel
-    m = StokesModel("Stokes3D_ve_perf", n=n, nt=nt, iter_max=10 ** 7, nout=200, norm_kind=1, eps=1e-8).setup()
+    m = StokesModel(preset, n=n, nt=nt, iter_max=10 ** 7, nout=200, norm_kind=1, eps=1e-8, **kw).setup()
     p = m.p
     m.ctx.sync()
     steps = []
@@ -184,21 +182,36 @@ def bench_c3_solve(n=511, nt=5, chunk=10000):
         while True:
             r = m.ctx.pt_solve(m.pt_params(), p["eps"], p["nout"], chunk - 1, p["norm_kind"], use_graph=True)
             iters += r["iters"]
-            print(f"# 3D VE {n}^3 step {it + 1}: {iters} iterations, err {r['err']:.3e}", file=sys.stderr,
-                  flush=True)
+            print(f"# {label} step {it + 1}: {iters} iterations, err {r['err']:.3e}", file=sys.stderr, flush=True)
             if r["converged"]:
                 break
             if iters >= 40 * chunk:
-                raise RuntimeError(f"3D VE {n}^3 step {it + 1} did not converge in {iters} iterations")
+                raise RuntimeError(f"{label} step {it + 1} did not converge in {iters} iterations")
         steps.append(dict(iters=iters, s=round(time.perf_counter() - t0, 2), err=r["err"]))
     wall = time.perf_counter() - t_all
     total = sum(st["iters"] for st in steps)
+    place = m.ctx.placement_times()
     m.close()
-    a_eff = 29.0 * n ** 3 * 8
-    return dict(workload=f"3D Maxwell VE Stokes, nx=ny=nz={n}, FP64, {nt} physical steps, each to "
-                         "||res||_inf < 1e-8 (check every 200 iterations)",
-                steps=steps, iterations=total, wall_s=round(wall, 2), ms_per_it=round(wall * 1e3 / total, 4),
-                T_eff_GBs=round(a_eff * total / wall / 1e9, 1))
+    return dict(steps=steps, iterations=total, wall_s=round(wall, 2), ms_per_it=round(wall * 1e3 / total, 4),
+                T_eff_GBs=round(a_eff * total / wall / 1e9, 1), placement_ms=place)
+
+
+def bench_c3_solve(n=511, nt=5):
+    """The 3D Maxwell VE block (Stokes3D_ve_perf.jl kernels, G = 1, dt = 1,
+    x-z pure shear, 1e-3 sphere), FP64."""
+    out = bench_solve("Stokes3D_ve_perf", n, nt, 29.0 * n ** 3 * 8, f"3D VE {n}^3")
+    out["workload"] = (f"3D Maxwell VE Stokes, nx=ny=nz={n}, FP64, {nt} physical steps, each to "
+                       "||res||_inf < 1e-8 (check every 200 iterations)")
+    return out
+
+
+def bench_c2_solve(n=8191, nt=1):
+    """The 2D Maxwell VE configuration (Stokes2D_ve_perf.jl kernels, G = 1,
+    dt = η/G/10 = 0.1, pure shear, 1e-3 inclusion), FP64."""
+    out = bench_solve("Stokes2D_ve_perf", n, nt, 18.0 * n * n * 8, f"2D VE {n}^2", chunk=20000, dt=0.1)
+    out["workload"] = (f"2D Maxwell VE Stokes, nx=ny={n}, dt=0.1, FP64, {nt} physical steps, each to "
+                       "||res||_inf < 1e-8 (check every 200 iterations)")
+    return out
 
 
 def bench_c1(n=127, threads=16):
@@ -326,6 +339,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 127^2 and 2047^2 extra lines")
     ap.add_argument("--c3-solve-steps", type=int, default=0,
                     help="also run that many converged physical steps of the 3D VE block (minutes)")
+    ap.add_argument("--c2-solve-steps", type=int, default=0,
+                    help="also run that many converged physical steps of 2D VE 8191^2 (minutes each)")
     ap.add_argument("--tts-no-graph", action="store_true",
                     help="FP64 time-to-tolerance without HIP graphs (for tracing tools)")
     args = ap.parse_args()
@@ -364,6 +379,11 @@ def main():
             extra["c1_viscous_127_time_to_solution"] = c1
             print(f"# 127^2 viscous: GPU {c1['gpu_s']} s, CPU {c1['cpu_s']} s ({c1['gpu_iters']} iterations)",
                   file=sys.stderr, flush=True)
+    if args.c2_solve_steps > 0:
+        c2s = bench_c2_solve(args.n2d, args.c2_solve_steps)
+        extra["c2_2d_ve_time_to_solution"] = c2s
+        print(f"# 2D VE {args.n2d}^2 time to solution: {c2s['iterations']} iterations in {c2s['wall_s']} s",
+              file=sys.stderr, flush=True)
     if args.c3_solve_steps > 0:
         c3s = bench_c3_solve(args.n3d, args.c3_solve_steps)
         extra["c3_3d_ve_time_to_solution"] = c3s
