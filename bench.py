@@ -81,11 +81,16 @@ def bench_2d(n, steps, warmup, ev):
     # 2D Maxwell VE: G = η = 1, dt = η/G/10, pure shear εbg = 1, 1e-3 inclusion of radius 1
     m = StokesModel("Stokes2D_ve_perf", n=n, dt=0.1).setup()
     m.ctx.assign_tau()
+    t0 = time.perf_counter()
+    m.ctx.pt_iterate(m.pt_params(), 0)  # first call: placement of the field blocks, no iteration
+    m.ctx.sync()
+    place_s = round(time.perf_counter() - t0, 3)
     ms, wall = time_iterations(m, steps, warmup, ev)
     t_it = ms / 1e3 / steps
     a_eff = 18.0 * n * n * 8
     out = dict(n=n, steps=steps, ms_per_it=ms / steps, wall_ms_per_it=wall * 1e3 / steps, A_eff_bytes=a_eff,
-               T_eff_GBs=a_eff / t_it / 1e9, device_bytes=m.ctx.device_bytes())
+               T_eff_GBs=a_eff / t_it / 1e9, device_bytes=m.ctx.device_bytes(),
+               placement_ms=m.ctx.placement_times(), placement_s=place_s)
     m.close()
     return out
 
@@ -351,6 +356,8 @@ def main():
 
     ev = HipEvents()
     r2 = bench_2d(args.n2d, args.steps, args.warmup, ev)
+    print(f"# 2D VE {args.n2d}^2 placements tried (ms): {r2['placement_ms']} ({r2['placement_s']} s)",
+          file=sys.stderr)
     print(f"# 2D VE {args.n2d}^2: {r2['ms_per_it']:.4f} ms/it, T_eff {r2['T_eff_GBs']:.1f} GB/s", file=sys.stderr,
           flush=True)
     extra = {}
@@ -401,6 +408,7 @@ def main():
         cpu = cpu_baseline(args.n2d, args.cpu_iters, threads)
         print(f"# CPU baseline: {cpu['value']:.2f} GB/s ({cpu['sample']})", file=sys.stderr, flush=True)
 
+    extra["2d_ve_placement"] = dict(ms_per_it_by_set=r2["placement_ms"], s=r2["placement_s"])
     t_eff = r2["T_eff_GBs"]
     line = {
         "metric": METRIC,

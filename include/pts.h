@@ -90,14 +90,16 @@ int pts_sync(pts_ctx* ctx);
 int pts_get_stream(const pts_ctx* ctx, void** stream);
 /* Bytes of device memory held by the context. */
 int pts_device_bytes(const pts_ctx* ctx, int64_t* bytes);
-/* No counterpart in the reference (placement of this library's 3D device
- * blocks).  On its first pts_pt_iterate / pts_pt_solve, a 3D single-pass
- * context whose field blocks total at least PTS_3D_PLACE_MIN_MB (default
- * 4096) times its iteration kernel, then up to PTS_3D_PLACE_TRIES (default 3)
- * times moves the blocks to fresh allocations and keeps the faster set (the
- * kernel's speed depends on the blocks' physical placement).  ms[0] is the
- * kernel time on the original blocks, ms[t] that of try t (0: not run). */
-int pts_placement_times(const pts_ctx* ctx, float ms[8]);
+/* No counterpart in the reference (placement of this library's device
+ * blocks).  On its first pts_pt_iterate / pts_pt_solve, a single-pass context
+ * whose field blocks total at least PTS_PLACE_MIN_MB (default 4096) times
+ * real iterations on copies of its state in its own blocks and in up to
+ * PTS_3D_PLACE_TRIES (default 3) / PTS_2D_PLACE_TRIES (default 12) fresh sets
+ * of blocks, and keeps the fastest set with the unchanged state (the
+ * iteration speed depends on the blocks' physical placement).  Fills ms[0..n)
+ * with the iteration times of the sets tried: ms[0] the original blocks,
+ * ms[t] try t, 0 for sets not tried (at most 16). */
+int pts_placement_times(const pts_ctx* ctx, float* ms, int32_t n);
 
 /* ---- setup kernels ------------------------------------------------------- */
 /* Initial conditions of every reference script (Stokes2D_ve.jl:131-136,

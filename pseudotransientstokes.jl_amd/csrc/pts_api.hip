@@ -51,13 +51,15 @@ struct pts_ctx {
     int variant_3d = 1;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
     bool group_3d = true;       // 3D: state fields, partners and τ_o in grouped allocations (ensure_group)
     bool map_flat_3d = false;   // k3_iter2: tiles in the flat (chunk, tile) order
-    // 3D single-pass placement selection (place_3d): tries, the least bytes of
+    // placement selection (place_fields): tries (3D, 2D), the least bytes of
     // field blocks for which it runs, a forced outcome for tests (-1: by time)
     int place_tries_3d = 3;
+    int place_tries_2d = 12;
+    bool group_2d = false;      // 2D: state fields, partners and τ_o in grouped allocations
     int64_t place_min_bytes = (int64_t)4 << 30;
     int place_accept = -1;
-    bool placed_3d = false;
-    float place_ms[8] = {};     // k3 launch time of each placement tried (0: not tried)
+    bool placed = false;
+    float place_ms[16] = {};    // iteration time on each placement tried (0: not tried)
     // adjoint state λ (ping-pong by field id), the pass-1 scratch, the
     // source ∂J/∂Vx, ∂J/∂Vy and the gradient buffer
     double* adj[2][PTS_NFIELDS] = {};
@@ -207,7 +209,7 @@ int ensure_group(pts_ctx* c, int f) {
 int ensure(pts_ctx* c, int f) {
     if (!field_valid(c, f)) return fail(c, PTS_ERR_STATE, "field " + std::to_string(f) + " not valid for this grid");
     if (c->cur[f]) return PTS_OK;
-    if (c->nd == 3 && c->group_3d) {
+    if ((c->nd == 3 && c->group_3d) || (c->nd == 2 && c->group_2d)) {
         if (f >= PTS_PT && f <= PTS_TYZ) return ensure_group(c, f);
         if (f >= PTS_TXX_O && f <= PTS_TYZ_O && c->g.ve) {
             const int s = f - (PTS_TXX_O - PTS_TXX);
@@ -353,9 +355,11 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_3D_VARIANT")) c->variant_3d = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_GROUPS")) c->group_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_MAP_FLAT")) c->map_flat_3d = std::atoi(e) != 0;
-    if (const char* e = std::getenv("PTS_3D_PLACE_TRIES")) c->place_tries_3d = std::min(7, std::max(0, std::atoi(e)));
-    if (const char* e = std::getenv("PTS_3D_PLACE_MIN_MB")) c->place_min_bytes = std::atoll(e) << 20;
-    if (const char* e = std::getenv("PTS_3D_PLACE_ACCEPT")) c->place_accept = std::atoi(e) != 0 ? 1 : 0;
+    if (const char* e = std::getenv("PTS_3D_PLACE_TRIES")) c->place_tries_3d = std::min(15, std::max(0, std::atoi(e)));
+    if (const char* e = std::getenv("PTS_2D_PLACE_TRIES")) c->place_tries_2d = std::min(15, std::max(0, std::atoi(e)));
+    if (const char* e = std::getenv("PTS_2D_GROUPS")) c->group_2d = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_PLACE_MIN_MB")) c->place_min_bytes = std::atoll(e) << 20;
+    if (const char* e = std::getenv("PTS_PLACE_ACCEPT")) c->place_accept = std::atoi(e) != 0 ? 1 : 0;
     if (const char* e = std::getenv("PTS_DERIVED_PARAMS")) c->dp_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_DERIVED_MUSTAU")) c->mx_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_ADJ_TWO_PASS")) c->adj_fused = std::atoi(e) == 0;
@@ -915,19 +919,20 @@ static int run_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_las
     return PTS_OK;
 }
 
-// ---- placement of the 3D field blocks ------------------------------------
-// The single-pass 3D kernel streams 28 arrays at once.  How evenly they load
-// the HBM channels depends on where the driver placed the allocations
-// physically: for one set of allocations the kernel's time is constant, but
-// across sets it takes one of (mainly) two levels, about 20 % apart at 511³.
-// place_3d times the kernel on the context's blocks, then up to
-// place_tries_3d times moves every block that holds a field to fresh
-// allocations (the current ones still held, so the new ones land elsewhere),
-// times it again and keeps the faster set (by more than 1 %: for one set the
-// time repeats to about 0.1 %).  The timed launches read the
-// current buffers and write the ping-pong partners without swapping them:
-// each launch recomputes the same partner values, which the next iteration
-// overwrites, so the state is unchanged.
+// ---- placement of the field blocks ----------------------------------------
+// The single-pass kernels stream 16 (2D VE) to 28 (3D VE) arrays at once.
+// How evenly they load the HBM channels depends on where the driver placed
+// the allocations physically: for one set of allocations the iteration time
+// is constant (to about 0.1 %), but across sets it differs by up to 20 %
+// (3D 511³: ≈6.1 or ≈7.3 ms; 2D 8191²: 1.56–1.85 ms once the device has seen
+// allocations come and go).  place_fields measures that and keeps a fast
+// set: it copies every block that holds a field into a spare set, times real
+// iterations (both ping-pong parities) on the context's blocks, then up to
+// place_tries_3d (2D: place_tries_2d) times allocates a fresh set (the
+// current ones still held, so it lands elsewhere), copies the spare into it,
+// times real iterations there and keeps it if it is more than 1 % faster.
+// The spare is finally copied into the kept set, so the state is what it
+// was before: the timed iterations only ever advance copies.
 
 struct Block {
     char* q;     // handed pointer (alloc_base key)
@@ -951,10 +956,10 @@ static std::vector<Block> field_blocks(const pts_ctx* c) {
 }
 
 // Every field pointer into from[i] moves to the same offset of to[i].
-static void repoint(pts_ctx* c, const std::vector<Block>& from, const std::vector<char*>& to) {
+static void repoint(pts_ctx* c, const std::vector<Block>& from, const std::vector<Block>& to) {
     auto moved = [&](void* x) -> void* {
         for (size_t i = 0; i < from.size(); ++i)
-            if (inside(from[i], x)) return to[i] + ((char*)x - from[i].p);
+            if (inside(from[i], x)) return to[i].p + ((char*)x - from[i].p);
         return x;
     };
     for (int q = 0; q < PTS_NFIELDS; ++q) {
@@ -967,104 +972,129 @@ static void repoint(pts_ctx* c, const std::vector<Block>& from, const std::vecto
     for (size_t i = 0; i < from.size(); ++i) {
         c->alloc_base.erase(from[i].q);
         c->alloc_len.erase(from[i].q);
-        char* nq = to[i] + (from[i].q - from[i].p);
-        c->alloc_base[nq] = to[i];
-        c->alloc_len[nq] = from[i].len;
+        c->alloc_base[to[i].q] = to[i].p;
+        c->alloc_len[to[i].q] = to[i].len;
     }
 }
 
-// Mean time of 3 single-pass launches (after one untimed) from the current
-// buffers into the partners, without swapping them.
-static int time_k3(pts_ctx* c, const StepScalars& s0, float* ms) {
-    const bool ve = c->g.ve != 0, mul = c->g.fd_mode == PTS_FD_MUL, bf = c->g.body_force != 0;
-    const StepScalars s = with_dp(c, s0);
+// A set of blocks shaped like `like` (same lengths and handed offsets), or
+// an empty vector when the device cannot hold it.
+static std::vector<Block> alloc_like(const std::vector<Block>& like) {
+    std::vector<Block> out;
+    for (const Block& b : like) {
+        void* p = nullptr;
+        if (hipMalloc(&p, b.len) != hipSuccess) {
+            for (const Block& x : out) (void)hipFree(x.p);
+            (void)hipGetLastError();
+            return {};
+        }
+        out.push_back(Block{(char*)p + (b.q - b.p), (char*)p, b.len});
+    }
+    return out;
+}
+
+static hipError_t copy_set(pts_ctx* c, const std::vector<Block>& dst, const std::vector<Block>& src) {
+    for (size_t i = 0; i < src.size(); ++i)
+        if (hipError_t e = hipMemcpyAsync(dst[i].p, src[i].p, src[i].len, hipMemcpyDeviceToDevice, c->stream))
+            return e;
+    return hipSuccess;
+}
+
+static void free_set(pts_ctx* c, const std::vector<Block>& set) {
+    (void)hipStreamSynchronize(c->stream);
+    for (const Block& b : set) (void)hipFree(b.p);
+}
+
+// Mean time of an even number of real iterations (after two untimed ones)
+// on the context's current blocks: 10 in 2D, 4 in 3D (a 2D iteration at
+// 8191² takes 1.6–1.8 ms, a 3D one at 511³ 6–7 ms).  The state pointers end
+// where they began; the fields have advanced.
+static int time_iter(pts_ctx* c, const StepScalars& s, float* ms) {
+    const int timed = c->nd == 2 ? 10 : 4;
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipError_t e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
-    for (int k = 0; k < 4 && e == hipSuccess; ++k) {
-        if (k == 1) e = hipEventRecord(ev[0], c->stream);
-        if (e == hipSuccess)
-            e = launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
-                             c->tile_rows_3d, use_dp(c), c->variant_3d, c->slot_bytes, use_mx(c), c->map_flat_3d);
-    }
-    if (e == hipSuccess) e = hipEventRecord(ev[1], c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    int rc = e == hipSuccess ? enqueue_iters(c, s, 2, false) : PTS_OK;
+    if (e == hipSuccess && rc == PTS_OK) e = hipEventRecord(ev[0], c->stream);
+    if (e == hipSuccess && rc == PTS_OK) rc = enqueue_iters(c, s, timed, false);
+    if (e == hipSuccess && rc == PTS_OK) e = hipEventRecord(ev[1], c->stream);
+    if (e == hipSuccess && rc == PTS_OK) e = hipEventSynchronize(ev[1]);
     float t = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+    if (e == hipSuccess && rc == PTS_OK) e = hipEventElapsedTime(&t, ev[0], ev[1]);
     for (hipEvent_t x : ev)
         if (x) (void)hipEventDestroy(x);
+    if (rc) return rc;
     if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("placement timing: ") + hipGetErrorString(e));
-    *ms = t / 3.f;
+    *ms = t / (float)timed;
     return PTS_OK;
 }
 
-static int place_3d(pts_ctx* c, const StepScalars& s) {
-    if (c->placed_3d) return PTS_OK;
-    c->placed_3d = true;
-    if (c->nd != 3 || !single_pass(c) || c->variant_3d < 1 || !c->group_3d || c->place_tries_3d <= 0) return PTS_OK;
+static int place_fields(pts_ctx* c, const StepScalars& s) {
+    if (c->placed) return PTS_OK;
+    c->placed = true;
+    const int tries = c->nd == 3 ? c->place_tries_3d : c->place_tries_2d;
+    if (!single_pass(c) || tries <= 0) return PTS_OK;
+    if (c->nd == 3 && (c->variant_3d < 1 || !c->group_3d)) return PTS_OK;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HIPCHK(c, hipStreamIsCapturing(c->stream, &cs));
     if (cs != hipStreamCaptureStatusNone) return PTS_OK;
-    std::vector<Block> blocks = field_blocks(c);
+    std::vector<Block> best = field_blocks(c);
     size_t bytes = 0;
-    for (const Block& b : blocks) bytes += b.len;
+    for (const Block& b : best) bytes += b.len;
     if ((int64_t)bytes < c->place_min_bytes) return PTS_OK;
-    float best = 0.f;
-    if (int rc = time_k3(c, s, &best)) return rc;
-    c->place_ms[0] = best;
-    bool moved_any = false;
-    for (int t = 1; t <= c->place_tries_3d; ++t) {
-        size_t avail = 0, total = 0;
-        if (hipMemGetInfo(&avail, &total) != hipSuccess || avail < bytes + ((size_t)1 << 30)) break;
-        std::vector<char*> to;
-        for (const Block& b : blocks) {
-            void* p = nullptr;
-            if (hipMalloc(&p, b.len) != hipSuccess) break;
-            to.push_back((char*)p);
-        }
-        hipError_t e = to.size() == blocks.size() ? hipSuccess : hipErrorOutOfMemory;
-        for (size_t i = 0; i < to.size() && e == hipSuccess; ++i)
-            e = hipMemcpyAsync(to[i], blocks[i].p, blocks[i].len, hipMemcpyDeviceToDevice, c->stream);
-        if (e != hipSuccess) {
-            (void)hipStreamSynchronize(c->stream);
-            for (char* p : to) (void)hipFree(p);
-            (void)hipGetLastError();
-            break;  // placement is best effort: keep the current blocks
-        }
-        std::vector<Block> fresh;
-        for (size_t i = 0; i < blocks.size(); ++i)
-            fresh.push_back(Block{to[i] + (blocks[i].q - blocks[i].p), to[i], blocks[i].len});
-        repoint(c, blocks, to);
-        float ms = 0.f;
-        const int rc = time_k3(c, s, &ms);
-        c->place_ms[t] = ms;
-        const bool keep = rc == PTS_OK && (c->place_accept >= 0 ? c->place_accept == 1 : ms < 0.99f * best);
-        std::vector<char*> gone;
-        if (keep) {
-            for (const Block& b : blocks) gone.push_back(b.p);
-            blocks = fresh;
-            best = ms;
-            moved_any = true;
-        } else {
-            std::vector<char*> back;
-            for (const Block& b : blocks) back.push_back(b.p);
-            repoint(c, fresh, back);
-            gone = to;
-        }
-        (void)hipStreamSynchronize(c->stream);
-        for (char* p : gone) (void)hipFree(p);
-        if (rc) return rc;
+    size_t avail = 0, total = 0;
+    if (hipMemGetInfo(&avail, &total) != hipSuccess || avail < 2 * bytes + ((size_t)1 << 30)) {
+        (void)hipGetLastError();
+        return PTS_OK;  // room for the spare and one candidate set is needed
     }
-    if (moved_any) {
+    const std::vector<Block> spare = alloc_like(best);
+    if (spare.empty()) return PTS_OK;
+    if (hipError_t e = copy_set(c, spare, best)) {
+        free_set(c, spare);
+        return fail(c, PTS_ERR_HIP, std::string("placement copy: ") + hipGetErrorString(e));
+    }
+    float tb = 0.f;
+    int rc = time_iter(c, s, &tb);
+    c->place_ms[0] = tb;
+    bool moved = false;
+    for (int t = 1; t <= tries && rc == PTS_OK; ++t) {
+        const std::vector<Block> cand = alloc_like(best);
+        if (cand.empty()) break;
+        hipError_t e = copy_set(c, cand, spare);
+        if (e != hipSuccess) {
+            free_set(c, cand);
+            rc = fail(c, PTS_ERR_HIP, std::string("placement copy: ") + hipGetErrorString(e));
+            break;
+        }
+        repoint(c, best, cand);
+        float tc = 0.f;
+        rc = time_iter(c, s, &tc);
+        c->place_ms[t] = tc;
+        const bool keep = rc == PTS_OK && (c->place_accept >= 0 ? c->place_accept == 1 : tc < 0.99f * tb);
+        if (keep) {
+            free_set(c, best);
+            best = cand;
+            tb = tc;
+            moved = true;
+        } else {
+            repoint(c, cand, best);
+            free_set(c, cand);
+        }
+    }
+    // the state before the timed iterations, into the kept set
+    hipError_t e = copy_set(c, best, spare);
+    free_set(c, spare);
+    if (e != hipSuccess && rc == PTS_OK) rc = fail(c, PTS_ERR_HIP, std::string("placement copy: ") + hipGetErrorString(e));
+    if (moved) {
         for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
         c->graphs.clear();
     }
-    return PTS_OK;
+    return rc;
 }
 
-int pts_placement_times(const pts_ctx* c, float ms[8]) {
-    if (!c || !ms) return PTS_ERR_ARG;
-    for (int q = 0; q < 8; ++q) ms[q] = c->place_ms[q];
+int pts_placement_times(const pts_ctx* c, float* ms, int32_t n) {
+    if (!c || !ms || n < 0) return PTS_ERR_ARG;
+    for (int q = 0; q < n; ++q) ms[q] = q < 16 ? c->place_ms[q] : 0.f;
     return PTS_OK;
 }
 
@@ -1073,7 +1103,7 @@ int pts_pt_iterate(pts_ctx* c, const pts_pt_params* p, int64_t niter) {
     if (int rc = setdev(c)) return rc;
     if (int rc = prepare_iter(c)) return rc;
     StepScalars s = scal(p->r, p->G, p->dt, p->dx, p->dy, c->nd == 3 ? p->dz : 1.0);
-    if (int rc = place_3d(c, s)) return rc;
+    if (int rc = place_fields(c, s)) return rc;
     return run_iters(c, s, niter, false, false);
 }
 
@@ -1101,7 +1131,7 @@ int pts_pt_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp,
     if (int rc = setdev(c)) return rc;
     if (int rc = prepare_iter(c)) return rc;
     StepScalars s = scal(p->r, p->G, p->dt, p->dx, p->dy, c->nd == 3 ? p->dz : 1.0);
-    if (int rc = place_3d(c, s)) return rc;
+    if (int rc = place_fields(c, s)) return rc;
     std::memset(res, 0, sizeof(*res));
     int64_t iter = 0;
     double err = 2.0 * sp->eps;

@@ -105,9 +105,9 @@ class Context:
         return b.value
 
     def placement_times(self):
-        """Kernel times (ms) of the 3D block placements tried: [original, try 1, ...]."""
-        ms = (ctypes.c_float * 8)()
-        self._chk(self.lib.pts_placement_times(self.h, ms))
+        """Iteration times (ms) on the block placements tried: [original, try 1, ...]."""
+        ms = (ctypes.c_float * 16)()
+        self._chk(self.lib.pts_placement_times(self.h, ms, 16))
         return [round(float(x), 4) for x in ms if x > 0]
 
     # -- setup kernels -------------------------------------------------------

@@ -134,31 +134,39 @@ def test_cached_graphs_follow_new_scalars():
     b.close()
 
 
+@pytest.mark.parametrize("dim", ["3D", "2D"])
 @pytest.mark.parametrize("accept", ["1", "0"])
-def test_moved_blocks_keep_state_and_graphs(accept, monkeypatch):
-    # a context whose field blocks are moved (or tried and moved back) before
-    # its first solve iterates exactly like one that keeps its blocks, through
-    # captured graphs, the per-call path and later downloads
-    monkeypatch.setenv("PTS_3D_PLACE_MIN_MB", "0")
-    monkeypatch.setenv("PTS_3D_PLACE_ACCEPT", accept)
-    monkeypatch.setenv("PTS_3D_PLACE_TRIES", "3")
-    a, _ = pair("Stokes3D_ve_perf", nx=45, ny=26, nz=21)
-    monkeypatch.setenv("PTS_3D_PLACE_TRIES", "0")
-    b, _ = pair("Stokes3D_ve_perf", nx=45, ny=26, nz=21)
-    nbytes = a.ctx.device_bytes()
+def test_moved_blocks_keep_state_and_graphs(accept, dim, monkeypatch):
+    # a context whose field blocks are timed with real iterations on copies
+    # and moved (or tried and moved back) before its first solve iterates
+    # exactly like one that keeps its blocks, through captured graphs, the
+    # per-call path and later downloads
+    tries = f"PTS_{dim}_PLACE_TRIES"
+    monkeypatch.setenv("PTS_PLACE_MIN_MB", "0")
+    monkeypatch.setenv("PTS_PLACE_ACCEPT", accept)
+    monkeypatch.setenv(tries, "3")
+    if dim == "3D":
+        case = ("Stokes3D_ve_perf", dict(nx=45, ny=26, nz=21))
+        fields = STATE + HIST + ["Mus"]
+    else:
+        case = ("Stokes2D_ve_perf", dict(nx=300, ny=170))
+        fields = ["Pt", "Vx", "Vy", "txx", "tyy", "txy", "txx_o", "tyy_o", "txy_o", "Mus"]
+    a, _ = pair(case[0], **case[1])
+    monkeypatch.setenv(tries, "0")
+    b, _ = pair(case[0], **case[1])
     params = a.pt_params()
     a.ctx.pt_solve(params, 1e-300, 10, 29, 0, use_graph=True)  # 30 iterations, no convergence
     b.ctx.pt_iterate(params, 30)
     assert len(a.ctx.placement_times()) == 4 and b.ctx.placement_times() == []
-    assert a.ctx.device_bytes() == nbytes
-    for f in STATE:
+    assert a.ctx.device_bytes() == b.ctx.device_bytes()
+    for f in fields:
         assert np.array_equal(a.get(f), b.get(f)), f
     a.ctx.assign_tau()
     b.ctx.assign_tau()
     a.ctx.pt_solve(params, 1e-300, 10, 19, 0, use_graph=True)
     a.ctx.pt_iterate(params, 3)
     b.ctx.pt_iterate(params, 23)
-    for f in STATE + HIST + ["Mus"]:
+    for f in fields:
         assert np.array_equal(a.get(f), b.get(f)), f
     a.close()
     b.close()
@@ -264,9 +272,12 @@ VARIANTS = [
     # placement selection moved into every size: blocks moved on every try,
     # never, or when faster
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12),
-     {"PTS_3D_PLACE_MIN_MB": "0", "PTS_3D_PLACE_ACCEPT": "1", "PTS_3D_PLACE_TRIES": "2"}),
-    ("Stokes3D_ve_multixpu", dict(n=15, rhog_in=-1.0), {"PTS_3D_PLACE_MIN_MB": "0", "PTS_3D_PLACE_ACCEPT": "0"}),
-    ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_3D_PLACE_MIN_MB": "0"}),
+     {"PTS_PLACE_MIN_MB": "0", "PTS_PLACE_ACCEPT": "1", "PTS_3D_PLACE_TRIES": "2"}),
+    ("Stokes3D_ve_multixpu", dict(n=15, rhog_in=-1.0), {"PTS_PLACE_MIN_MB": "0", "PTS_PLACE_ACCEPT": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_PLACE_MIN_MB": "0"}),
+    ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_PLACE_MIN_MB": "0", "PTS_2D_PLACE_TRIES": "2"}),
+    ("Stokes2D", dict(nx=253, ny=40, rhog_in=-1.0),
+     {"PTS_PLACE_MIN_MB": "0", "PTS_2D_PLACE_TRIES": "2", "PTS_PLACE_ACCEPT": "1"}),
 ]
 
 
