@@ -1058,6 +1058,7 @@ static int place_fields(pts_ctx* c, const StepScalars& s) {
     c->place_ms[0] = tb;
     bool moved = false;
     for (int t = 1; t <= tries && rc == PTS_OK; ++t) {
+        if (hipMemGetInfo(&avail, &total) != hipSuccess || avail < bytes + ((size_t)4 << 30)) break;
         const std::vector<Block> cand = alloc_like(best);
         if (cand.empty()) break;
         hipError_t e = copy_set(c, cand, spare);
