@@ -76,22 +76,34 @@ def time_iterations(model, steps, warmup, ev):
     return ev.elapsed_ms(a, b), wall
 
 
-def bench_2d(n, steps, warmup, ev):
+def bench_2d(n, steps, warmup, ev, contexts=1):
+    """2D VE iteration time on `contexts` contexts created one after another
+    (each freed before the next).  As in 3D, the kernel's HBM rate depends on
+    where the arrays land physically; the library selects among a few
+    placements on a context's first iteration call (timed on its own, outside
+    the timed region), and what it finds depends on the device's history, so
+    the median context is reported with all of them."""
     from ptstokes import StokesModel
-    # 2D Maxwell VE: G = η = 1, dt = η/G/10, pure shear εbg = 1, 1e-3 inclusion of radius 1
-    m = StokesModel("Stokes2D_ve_perf", n=n, dt=0.1).setup()
-    m.ctx.assign_tau()
-    t0 = time.perf_counter()
-    m.ctx.pt_iterate(m.pt_params(), 0)  # first call: placement of the field blocks, no iteration
-    m.ctx.sync()
-    place_s = round(time.perf_counter() - t0, 3)
-    ms, wall = time_iterations(m, steps, warmup, ev)
-    t_it = ms / 1e3 / steps
     a_eff = 18.0 * n * n * 8
-    out = dict(n=n, steps=steps, ms_per_it=ms / steps, wall_ms_per_it=wall * 1e3 / steps, A_eff_bytes=a_eff,
-               T_eff_GBs=a_eff / t_it / 1e9, device_bytes=m.ctx.device_bytes(),
-               placement_ms=m.ctx.placement_times(), placement_s=place_s)
-    m.close()
+    runs = []
+    for _ in range(max(1, contexts)):
+        # 2D Maxwell VE: G = η = 1, dt = η/G/10, pure shear εbg = 1, 1e-3 inclusion of radius 1
+        m = StokesModel("Stokes2D_ve_perf", n=n, dt=0.1).setup()
+        m.ctx.assign_tau()
+        t0 = time.perf_counter()
+        m.ctx.pt_iterate(m.pt_params(), 0)  # first call: placement of the field blocks, no iteration
+        m.ctx.sync()
+        place_s = round(time.perf_counter() - t0, 3)
+        ms, wall = time_iterations(m, steps, warmup, ev)
+        runs.append(dict(ms_per_it=ms / steps, wall_ms_per_it=wall * 1e3 / steps, device_bytes=m.ctx.device_bytes(),
+                         placement_ms=m.ctx.placement_times(), placement_s=place_s))
+        m.close()
+    order = sorted(range(len(runs)), key=lambda i: runs[i]["ms_per_it"])
+    out = dict(runs[order[len(order) // 2]])
+    out.update(n=n, steps=steps, A_eff_bytes=a_eff, T_eff_GBs=a_eff / (out["ms_per_it"] / 1e3) / 1e9,
+               ms_per_it_by_context=[round(r["ms_per_it"], 4) for r in runs],
+               placement_ms_by_context=[r["placement_ms"] for r in runs],
+               placement_s_by_context=[r["placement_s"] for r in runs])
     return out
 
 
@@ -335,6 +347,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n2d", type=int, default=8191)
     ap.add_argument("--n3d", type=int, default=511)
+    ap.add_argument("--contexts2d", type=int, default=3, help="2D: contexts timed one after another (median reported)")
     ap.add_argument("--contexts3d", type=int, default=3, help="3D: fresh allocations timed (median reported)")
     ap.add_argument("--no-3d", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -355,9 +368,9 @@ def main():
         return 2
 
     ev = HipEvents()
-    r2 = bench_2d(args.n2d, args.steps, args.warmup, ev)
-    print(f"# 2D VE {args.n2d}^2 placements tried (ms): {r2['placement_ms']} ({r2['placement_s']} s)",
-          file=sys.stderr)
+    r2 = bench_2d(args.n2d, args.steps, args.warmup, ev, args.contexts2d)
+    print(f"# 2D VE {args.n2d}^2 placements tried (ms): {r2['placement_ms_by_context']} "
+          f"({r2['placement_s_by_context']} s); contexts {r2['ms_per_it_by_context']} ms/it", file=sys.stderr)
     print(f"# 2D VE {args.n2d}^2: {r2['ms_per_it']:.4f} ms/it, T_eff {r2['T_eff_GBs']:.1f} GB/s", file=sys.stderr,
           flush=True)
     extra = {}
@@ -408,7 +421,9 @@ def main():
         cpu = cpu_baseline(args.n2d, args.cpu_iters, threads)
         print(f"# CPU baseline: {cpu['value']:.2f} GB/s ({cpu['sample']})", file=sys.stderr, flush=True)
 
-    extra["2d_ve_placement"] = dict(ms_per_it_by_set=r2["placement_ms"], s=r2["placement_s"])
+    extra["2d_ve_contexts"] = dict(ms_per_it_by_context=r2["ms_per_it_by_context"],
+                                   placement_ms_by_context=r2["placement_ms_by_context"],
+                                   placement_s_by_context=r2["placement_s_by_context"])
     t_eff = r2["T_eff_GBs"]
     line = {
         "metric": METRIC,
