@@ -94,11 +94,12 @@ int pts_device_bytes(const pts_ctx* ctx, int64_t* bytes);
  * blocks).  On its first pts_pt_iterate / pts_pt_solve, a single-pass context
  * whose field blocks total at least PTS_PLACE_MIN_MB (default 4096) times
  * real iterations on copies of its state in its own blocks and in up to
- * PTS_3D_PLACE_TRIES (default 3) / PTS_2D_PLACE_TRIES (default 12) fresh sets
- * of blocks, and keeps the fastest set with the unchanged state (the
- * iteration speed depends on the blocks' physical placement).  Fills ms[0..n)
- * with the iteration times of the sets tried: ms[0] the original blocks,
- * ms[t] try t, 0 for sets not tried (at most 16). */
+ * PTS_3D_PLACE_TRIES (default 5) / PTS_2D_PLACE_TRIES (default 31) fresh sets
+ * of blocks, within a time budget of PTS_PLACE_BUDGET_ITERS (default 5000)
+ * iterations (at least 1 s), and keeps the fastest set with the unchanged
+ * state (the iteration speed depends on the blocks' physical placement).
+ * Fills ms[0..n) with the iteration times of the sets tried: ms[0] the
+ * original blocks, ms[t] try t, 0 for sets not tried (at most 32). */
 int pts_placement_times(const pts_ctx* ctx, float* ms, int32_t n);
 
 /* ---- setup kernels ------------------------------------------------------- */

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -53,13 +54,14 @@ struct pts_ctx {
     bool map_flat_3d = false;   // k3_iter2: tiles in the flat (chunk, tile) order
     // placement selection (place_fields): tries (3D, 2D), the least bytes of
     // field blocks for which it runs, a forced outcome for tests (-1: by time)
-    int place_tries_3d = 3;
-    int place_tries_2d = 12;
+    int place_tries_3d = 5;
+    int place_tries_2d = 31;
+    int64_t place_budget_iters = 5000;  // stop trying after this many iterations' worth of time
     bool group_2d = false;      // 2D: state fields, partners and τ_o in grouped allocations
     int64_t place_min_bytes = (int64_t)4 << 30;
     int place_accept = -1;
     bool placed = false;
-    float place_ms[16] = {};    // iteration time on each placement tried (0: not tried)
+    float place_ms[32] = {};    // iteration time on each placement tried (0: not tried)
     // adjoint state λ (ping-pong by field id), the pass-1 scratch, the
     // source ∂J/∂Vx, ∂J/∂Vy and the gradient buffer
     double* adj[2][PTS_NFIELDS] = {};
@@ -355,11 +357,12 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_3D_VARIANT")) c->variant_3d = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_GROUPS")) c->group_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_MAP_FLAT")) c->map_flat_3d = std::atoi(e) != 0;
-    if (const char* e = std::getenv("PTS_3D_PLACE_TRIES")) c->place_tries_3d = std::min(15, std::max(0, std::atoi(e)));
-    if (const char* e = std::getenv("PTS_2D_PLACE_TRIES")) c->place_tries_2d = std::min(15, std::max(0, std::atoi(e)));
+    if (const char* e = std::getenv("PTS_3D_PLACE_TRIES")) c->place_tries_3d = std::min(31, std::max(0, std::atoi(e)));
+    if (const char* e = std::getenv("PTS_2D_PLACE_TRIES")) c->place_tries_2d = std::min(31, std::max(0, std::atoi(e)));
     if (const char* e = std::getenv("PTS_2D_GROUPS")) c->group_2d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_PLACE_MIN_MB")) c->place_min_bytes = std::atoll(e) << 20;
     if (const char* e = std::getenv("PTS_PLACE_ACCEPT")) c->place_accept = std::atoi(e) != 0 ? 1 : 0;
+    if (const char* e = std::getenv("PTS_PLACE_BUDGET_ITERS")) c->place_budget_iters = std::max<int64_t>(0, std::atoll(e));
     if (const char* e = std::getenv("PTS_DERIVED_PARAMS")) c->dp_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_DERIVED_MUSTAU")) c->mx_enabled = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_ADJ_TWO_PASS")) c->adj_fused = std::atoi(e) == 0;
@@ -1047,6 +1050,7 @@ static int place_fields(pts_ctx* c, const StepScalars& s) {
         (void)hipGetLastError();
         return PTS_OK;  // room for the spare and one candidate set is needed
     }
+    const auto t_start = std::chrono::steady_clock::now();
     const std::vector<Block> spare = alloc_like(best);
     if (spare.empty()) return PTS_OK;
     if (hipError_t e = copy_set(c, spare, best)) {
@@ -1057,7 +1061,13 @@ static int place_fields(pts_ctx* c, const StepScalars& s) {
     int rc = time_iter(c, s, &tb);
     c->place_ms[0] = tb;
     bool moved = false;
+    // time budget: place_budget_iters iterations at the original blocks' speed
+    // (at least 1 s); later tries take longer (fresh blocks from memory that
+    // other allocations used are cleared by the driver first)
+    const double budget_s = std::max(1.0, (double)c->place_budget_iters * tb * 1e-3);
     for (int t = 1; t <= tries && rc == PTS_OK; ++t) {
+        const double spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        if (spent > budget_s) break;
         if (hipMemGetInfo(&avail, &total) != hipSuccess || avail < bytes + ((size_t)4 << 30)) break;
         const std::vector<Block> cand = alloc_like(best);
         if (cand.empty()) break;
@@ -1095,7 +1105,7 @@ static int place_fields(pts_ctx* c, const StepScalars& s) {
 
 int pts_placement_times(const pts_ctx* c, float* ms, int32_t n) {
     if (!c || !ms || n < 0) return PTS_ERR_ARG;
-    for (int q = 0; q < n; ++q) ms[q] = q < 16 ? c->place_ms[q] : 0.f;
+    for (int q = 0; q < n; ++q) ms[q] = q < 32 ? c->place_ms[q] : 0.f;
     return PTS_OK;
 }
 

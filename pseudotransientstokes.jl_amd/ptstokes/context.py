@@ -106,8 +106,8 @@ class Context:
 
     def placement_times(self):
         """Iteration times (ms) on the block placements tried: [original, try 1, ...]."""
-        ms = (ctypes.c_float * 16)()
-        self._chk(self.lib.pts_placement_times(self.h, ms, 16))
+        ms = (ctypes.c_float * 32)()
+        self._chk(self.lib.pts_placement_times(self.h, ms, 32))
         return [round(float(x), 4) for x in ms if x > 0]
 
     # -- setup kernels -------------------------------------------------------
