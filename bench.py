@@ -231,6 +231,56 @@ def bench_c2_solve(n=8191, nt=1):
     return out
 
 
+def bench_c4_kernels(n, ev):
+    """Iteration kernels of the 3D viscous random-viscosity problem (log10 μ
+    a seeded Gaussian random field, 1e-3 sphere with ρg_z = -1) at full size:
+    the FP64 PT iteration (HIP events over 10 iterations after 2), and one
+    FP32 correction sweep of the defect correction (the difference of a
+    120-sweep and a 20-sweep correction, after a warm-up correction that
+    allocates the FP32 arrays).  The FP32 context keeps its FP64 arrays
+    ungrouped (no ping-pong partners), so that both precisions' arrays fit."""
+    from ptstokes import StokesModel
+    a_eff = 24.0 * n ** 3 * 8
+    m = StokesModel("Stokes3D_grf", n=n).setup()
+    p = m.pt_params()
+    m.ctx.pt_iterate(p, 2)
+    m.ctx.sync()
+    s = m.ctx.stream()
+    a, b = ev.event(), ev.event()
+    ev.record(a, s)
+    m.ctx.pt_iterate(p, 10)
+    ev.record(b, s)
+    m.ctx.sync()
+    ms64 = ev.elapsed_ms(a, b) / 10
+    bytes64 = m.ctx.device_bytes()
+    m.close()
+    print(f"# 3D GRF {n}^3 FP64 iteration {ms64:.3f} ms", file=sys.stderr, flush=True)
+    old = os.environ.get("PTS_3D_GROUPS")
+    os.environ["PTS_3D_GROUPS"] = "0"
+    try:
+        m = StokesModel("Stokes3D_grf", n=n).setup()
+    finally:
+        if old is None:
+            os.environ.pop("PTS_3D_GROUPS", None)
+        else:
+            os.environ["PTS_3D_GROUPS"] = old
+    ts = []
+    for k in (20, 20, 120):
+        t0 = time.perf_counter()
+        m.ctx.dc_solve(p, 1e-30, 1, max_outer=1, rtol_inner=0.0, nout_inner=k, iter_max_inner=k)
+        m.ctx.sync()
+        ts.append(time.perf_counter() - t0)
+    ms32 = (ts[2] - ts[1]) / 100 * 1e3
+    bytes32 = m.ctx.device_bytes()
+    m.close()
+    print(f"# 3D GRF {n}^3 FP32 correction sweep {ms32:.3f} ms", file=sys.stderr, flush=True)
+    return dict(workload=f"3D viscous Stokes, log10 mu GRF + 1e-3 sphere with rhog_z=-1, nx=ny=nz={n}",
+                fp64_ms_per_it=round(ms64, 4), fp64_A_eff_bytes=a_eff, fp64_T_eff_GBs=round(a_eff / ms64 / 1e6, 1),
+                fp32_ms_per_sweep=round(ms32, 4), fp32_A_eff_bytes=a_eff / 2,
+                fp32_T_eff_GBs=round(a_eff / 2 / ms32 / 1e6, 1), sweep_over_iteration=round(ms32 / ms64, 3),
+                device_bytes_fp64=bytes64, device_bytes_fp32_context=bytes32)
+
+
 def bench_c1(n=127, threads=16):
     """2D viscous Stokes (Stokes2D.jl, divide form) at 127²: 1e-3 inclusion of
     radius 1 with ρg_y = -1 inside, free-slip walls (εbg = 0), Re and r at the
@@ -357,6 +407,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 127^2 and 2047^2 extra lines")
     ap.add_argument("--c3-solve-steps", type=int, default=0,
                     help="also run that many converged physical steps of the 3D VE block (minutes)")
+    ap.add_argument("--c4-kernels", type=int, default=0,
+                    help="also time the FP64 iteration and FP32 sweep of the 3D random-viscosity problem at this n "
+                         "(1023: ~225 GB of device memory)")
     ap.add_argument("--c2-solve-steps", type=int, default=0,
                     help="also run that many converged physical steps of 2D VE 8191^2 (minutes each)")
     ap.add_argument("--tts-no-graph", action="store_true",
@@ -399,6 +452,8 @@ def main():
             extra["c1_viscous_127_time_to_solution"] = c1
             print(f"# 127^2 viscous: GPU {c1['gpu_s']} s, CPU {c1['cpu_s']} s ({c1['gpu_iters']} iterations)",
                   file=sys.stderr, flush=True)
+    if args.c4_kernels > 0:
+        extra["c4_kernels"] = bench_c4_kernels(args.c4_kernels, ev)
     if args.c2_solve_steps > 0:
         c2s = bench_c2_solve(args.n2d, args.c2_solve_steps)
         extra["c2_2d_ve_time_to_solution"] = c2s
