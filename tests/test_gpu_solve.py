@@ -96,3 +96,31 @@ def test_bad_arguments_raise():
     with pytest.raises(PtsError):
         c.bc_z("Pt")  # 3D kernel on a 2D grid
     c.close()
+
+
+@pytest.mark.parametrize("preset,kw", [("Stokes3D_ve_perf", dict(n=23)), ("Stokes2D_ve_perf", dict(n=95, dt=0.1))])
+def test_chunked_solve_equals_one_loop(preset, kw):
+    # the time-to-solution runs drive pts_pt_solve in chunks whose length is a
+    # multiple of the check interval: iteration count, final err and fields
+    # are those of one loop
+    opts = dict(iter_max=10 ** 6, nout=200, norm_kind=1, eps=1e-8, **kw)
+    a = StokesModel(preset, **opts).setup()
+    b = StokesModel(preset, **opts).setup()
+    a.ctx.assign_tau()
+    b.ctx.assign_tau()
+    ra = a.ctx.pt_solve(a.pt_params(), 1e-8, 200, 10 ** 6, 1)
+    assert ra["converged"]
+    iters = 0
+    while True:
+        rb = b.ctx.pt_solve(b.pt_params(), 1e-8, 200, 1000 - 1, 1)
+        iters += rb["iters"]
+        if rb["converged"]:
+            break
+        assert iters < 10 ** 6
+    assert iters == ra["iters"] and rb["err"] == ra["err"], (iters, ra)
+    for f in STATE:
+        if a.p["ndim"] == 2 and f in ("Vz", "tzz", "txz", "tyz"):
+            continue
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
