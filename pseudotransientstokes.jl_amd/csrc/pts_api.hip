@@ -933,9 +933,10 @@ static int run_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_las
 // iterations (both ping-pong parities) on the context's blocks, then up to
 // place_tries_3d (2D: place_tries_2d) times allocates a fresh set (the
 // current ones still held, so it lands elsewhere), copies the spare into it,
-// times real iterations there and keeps it if it is more than 1 % faster.
-// The spare is finally copied into the kept set, so the state is what it
-// was before: the timed iterations only ever advance copies.
+// times real iterations there and keeps it if it is more than 1 % faster,
+// within a time budget of place_budget_iters iterations.  The timed
+// iterations advance whichever set they run on; the spare, copied into the
+// kept set at the end, restores the state from before them.
 
 struct Block {
     char* q;     // handed pointer (alloc_base key)
