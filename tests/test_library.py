@@ -46,3 +46,16 @@ def test_field_names_match_header_enum():
 def test_package_loads_native_library():
     lib = ptstokes.load()
     assert lib.pts_last_error(None) is not None
+
+
+def test_missing_library_raises():
+    with pytest.raises(RuntimeError, match="not built"):
+        _lib.load(os.path.join(ROOT, "no_such_dir", "libpts.so"))
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present: creation succeeds")
+def test_no_gpu_means_error_not_fallback():
+    # Without a device the library reports a HIP error; nothing computes on the host.
+    with pytest.raises(ptstokes.PtsError) as e:
+        ptstokes.Context(2, 31, 31)
+    assert e.value.code == _lib.PTS_ERR_HIP
