@@ -67,12 +67,37 @@ function download(c::Ctx, f::Symbol)
 end
 copy!(c::Ctx, dst::Symbol, src::Symbol) = chk(c, ccall((:pts_copy, libpts), Cint, (Ptr{Cvoid}, Cint, Cint), c.h, fid(dst), fid(src)))
 swap!(c::Ctx, a::Symbol, b::Symbol) = chk(c, ccall((:pts_swap, libpts), Cint, (Ptr{Cvoid}, Cint, Cint), c.h, fid(a), fid(b)))
+# @zeros / @ones: A .= v on the device
+fill!(c::Ctx, f::Symbol, v) = chk(c, ccall((:pts_fill, libpts), Cint, (Ptr{Cvoid}, Cint, Float64), c.h, fid(f), v))
+sync(c::Ctx) = chk(c, ccall((:pts_sync, libpts), Cint, (Ptr{Cvoid},), c.h))
+# the context's hipStream_t, for callers that order their own work against it
+function stream(c::Ctx)
+    s = Ref{Ptr{Cvoid}}(C_NULL)
+    chk(c, ccall((:pts_get_stream, libpts), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), c.h, s))
+    return s[]
+end
+function device_bytes(c::Ctx)
+    b = Ref{Int64}(0)
+    chk(c, ccall((:pts_device_bytes, libpts), Cint, (Ptr{Cvoid}, Ref{Int64}), c.h, b))
+    return b[]
+end
+# iteration times (ms) of the field placements tried at the first solve
+function placement_times(c::Ctx, n=32)
+    ms = zeros(Float32, n)
+    chk(c, ccall((:pts_placement_times, libpts), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int32), c.h, ms, n))
+    return ms
+end
 
 # setup kernels
 init_inclusion!(c::Ctx, lx, ly, lz, μs0, μsi, rad2, εbg, ρg_in=0.0, ρg_out=0.0; igg=false) =
     chk(c, ccall((:pts_init_inclusion, libpts), Cint,
                  (Ptr{Cvoid}, Float64, Float64, Float64, Float64, Float64, Float64, Float64, Float64, Float64, Int32),
                  c.h, lx, ly, lz, μs0, μsi, rad2, εbg, ρg_in, ρg_out, igg))
+# seeded Gaussian random field for log10 μ: modes is a 4×n matrix of (kx, ky, kz, φ)
+init_grf_viscosity!(c::Ctx, modes::Matrix{Float64}, σ, lo, hi, lx, ly, lz, μsi, rad2) =
+    chk(c, ccall((:pts_init_grf_viscosity, libpts), Cint,
+                 (Ptr{Cvoid}, Int32, Ptr{Float64}, Float64, Float64, Float64, Float64, Float64, Float64, Float64, Float64),
+                 c.h, size(modes, 2), modes, σ, lo, hi, lx, ly, lz, μsi, rad2))
 smooth!(c::Ctx, A2::Symbol, A::Symbol, fact) =
     chk(c, ccall((:pts_smooth, libpts), Cint, (Ptr{Cvoid}, Cint, Cint, Float64), c.h, fid(A2), fid(A), fact))
 compute_maxloc!(c::Ctx, A2::Symbol, A::Symbol) =
@@ -104,6 +129,26 @@ compute_V_perf!(c::Ctx, _dx, _dy, _dz=1.0) =
     chk(c, ccall((:pts_compute_V_perf, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64), c.h, _dx, _dy, _dz))
 compute_Res_perf!(c::Ctx, _dx, _dy, _dz=1.0) =
     chk(c, ccall((:pts_compute_Res_perf, libpts), Cint, (Ptr{Cvoid}, Float64, Float64, Float64), c.h, _dx, _dy, _dz))
+
+# residual norms of the stored Rx, Ry, Rz, ∇V and the ranges of Vx and Pt
+# (Stokes2D_ve.jl:164-168); err from them (kind 0: the scripts' scaled RMS,
+# kind 1: max |R|)
+mutable struct Norms
+    Vmin::Float64; Vmax::Float64; Pmin::Float64; Pmax::Float64
+    sumsq::NTuple{4,Float64}; maxabs::NTuple{4,Float64}; len::NTuple{4,Int64}
+end
+Norms() = Norms(0.0, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 0.0), (0, 0, 0, 0))
+function norms(c::Ctx)
+    n = Ref(Norms())
+    chk(c, ccall((:pts_norms_stored, libpts), Cint, (Ptr{Cvoid}, Ref{Norms}), c.h, n))
+    return n[]
+end
+function err_from_norms(n::Norms, ndim, lx; kind=0)
+    comp = zeros(Float64, 4)
+    err = ccall((:pts_err_from_norms, libpts), Float64, (Ref{Norms}, Int32, Float64, Int32, Ptr{Float64}),
+                Ref(n), ndim, lx, kind, comp)
+    return err, comp
+end
 
 # device-resident PT loop of one physical step
 function pt_solve!(c::Ctx, p::PtParams, ε, nout, iterMax; norm_kind=0, use_graph=true)
@@ -144,6 +189,17 @@ function adjoint_set_source!(c::Ctx, dJ_dVx::Union{Nothing,Array{Float64}}, dJ_d
         (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Float64}, Int64), c.h, px,
         dJ_dVx === nothing ? 0 : length(dJ_dVx), py, dJ_dVy === nothing ? 0 : length(dJ_dVy)))
 end
+# λ of :Pt, :τxx, :τyy, :τxy, :Vx or :Vy (forward shapes)
+adjoint_upload!(c::Ctx, f::Symbol, A::Array{Float64}) =
+    chk(c, ccall((:pts_adjoint_upload, libpts), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64), c.h, fid(f), A, length(A)))
+function adjoint_download(c::Ctx, f::Symbol)
+    A = zeros(Float64, shape(c, f))
+    chk(c, ccall((:pts_adjoint_download, libpts), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Int64), c.h, fid(f), A, length(A)))
+    return A
+end
+# n applications of λ <- Φᵀλ + ∂J/∂x
+adjoint_iterate!(c::Ctx, p::PtParams, n) =
+    chk(c, ccall((:pts_adjoint_iterate, libpts), Cint, (Ptr{Cvoid}, Ref{PtParams}, Int64), c.h, Ref(p), n))
 function adjoint_solve!(c::Ctx, p::PtParams, ε, nout, iterMax)
     sp = Ref(SolveParams(ε, nout, iterMax, 0, 0))
     res = Ref(SolveResult())

@@ -59,3 +59,10 @@ def test_no_gpu_means_error_not_fallback():
     with pytest.raises(ptstokes.PtsError) as e:
         ptstokes.Context(2, 31, 31)
     assert e.value.code == _lib.PTS_ERR_HIP
+
+
+def test_julia_binding_ccalls_every_header_symbol():
+    # Julia is absent here, so this checks the binding's text: each entry point is ccall'ed.
+    jl = open(os.path.join(ROOT, "pseudotransientstokes.jl_amd", "julia", "PTStokesAMD.jl")).read()
+    called = set(re.findall(r"ccall\(\(:(pts_\w+),", jl))
+    assert called == set(header_symbols())
