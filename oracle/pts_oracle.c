@@ -22,8 +22,9 @@
  *     the backward-Euler Maxwell recursion for VE), x<->y antisymmetry of P in
  *     pure shear, and a sparse direct solve of the steady discrete equations.
  *   The body-force term (Stokes2D_param.jl:59) and the max-norm stop criterion
- *   have no reference fixture; they are pinned by the direct solve and the
- *   closed forms only.
+ *   have no reference fixture: the body force is pinned by the direct solve and
+ *   the hydrostatic closed form, and both err forms (scaled RMS and max-norm)
+ *   by hand-computed values on planted residuals (tests/test_err_norms.py).
  */
 #include "pts_oracle.h"
 
