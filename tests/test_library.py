@@ -66,3 +66,53 @@ def test_julia_binding_ccalls_every_header_symbol():
     jl = open(os.path.join(ROOT, "pseudotransientstokes.jl_amd", "julia", "PTStokesAMD.jl")).read()
     called = set(re.findall(r"ccall\(\(:(pts_\w+),", jl))
     assert called == set(header_symbols())
+
+
+def _zero_args(argtypes):
+    vals = []
+    for a in argtypes:
+        if a is ctypes.c_double:
+            vals.append(0.0)
+        elif a in (ctypes.c_int, ctypes.c_int32, ctypes.c_int64):
+            vals.append(0)
+        else:
+            vals.append(None)
+    return vals
+
+
+def test_every_entry_point_rejects_a_null_context():
+    # error codes, never a crash: checked before any device call
+    lib = ptstokes.load()
+    for name, _, args in _lib.SIGNATURES:
+        if name in ("pts_last_error", "pts_err_from_norms"):
+            continue
+        rc = getattr(lib, name)(*_zero_args(args))
+        want = _lib.PTS_OK if name == "pts_destroy" else _lib.PTS_ERR_ARG
+        assert rc == want, (name, rc)
+
+
+@pytest.mark.parametrize("grid,msg", [
+    ((4, 31, 31, 1), "ndim"),
+    ((2, 2, 31, 1), ">= 3"),
+    ((3, 31, 31, 2), ">= 3"),
+    ((2, 65001, 31, 1), "too large"),
+    ((2, 65000, 65000, 1), "32-bit"),
+    ((3, 1400, 1400, 1400), "32-bit"),
+])
+def test_create_rejects_bad_grids(grid, msg):
+    # argument checks come before device initialisation, so they run without a GPU
+    lib = ptstokes.load()
+    ndim, nx, ny, nz = grid
+    g = _lib.Grid(ndim, nx, ny, nz, 1, _lib.PTS_FD_MUL, 0, 0)
+    h = ctypes.c_void_p()
+    assert lib.pts_create(ctypes.byref(g), ctypes.byref(h)) == _lib.PTS_ERR_ARG
+    assert msg in lib.pts_last_error(None).decode()
+    assert not h.value
+
+
+def test_create_rejects_bad_fd_mode():
+    lib = ptstokes.load()
+    g = _lib.Grid(2, 31, 31, 1, 1, 7, 0, 0)
+    h = ctypes.c_void_p()
+    assert lib.pts_create(ctypes.byref(g), ctypes.byref(h)) == _lib.PTS_ERR_ARG
+    assert "fd_mode" in lib.pts_last_error(None).decode()
