@@ -271,12 +271,13 @@ def test_rrule_pullback_and_points():
     inv2.close()
 
 
-def test_ve_gradient_through_checkpointed_steps():
-    """Visco-elastic, 4 physical steps (each step's τ is the next step's τ_o):
-    the checkpointed backward pass against central differences of the whole
-    forward time loop (device forward, bit-identical to the oracle's; the
-    oracle reproduces J at the base point)."""
-    n, h, feps, nt = 48, 1e-5, 1e-12, 4
+@pytest.mark.parametrize("nt", [4, 10])
+def test_ve_gradient_through_checkpointed_steps(nt):
+    """Visco-elastic, nt physical steps (each step's τ is the next step's τ_o;
+    10 is the inversion's time loop): the checkpointed backward pass against
+    central differences of the whole forward time loop (device forward,
+    bit-identical to the oracle's; the oracle reproduces J at the base point)."""
+    n, h, feps = 48, 1e-5, 1e-12
     inv = Inversion(n=n, preset="Stokes2D_ve_grf", nt=nt, eps=feps, adj_eps=1e-14, nout=100, npts=40)
     inv.observe(inv.grf_logmu(4321))
     m0 = inv.grf_logmu(1234)
