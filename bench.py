@@ -157,6 +157,7 @@ def bench_tts(n, nout=500, rtol_inner=1e-4, use_graph=True):
     r64 = m.step(use_graph=use_graph)
     m.ctx.sync()
     t64 = time.perf_counter() - t0
+    print(f"# TTS {n}^3 FP64 PT: {t64:.2f} s, {r64['iters']} iterations", file=sys.stderr, flush=True)
     ref = {f: m.get(f) for f in names}
     m.close()
     m = StokesModel("Stokes3D_grf", n=n).setup()
