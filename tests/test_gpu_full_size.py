@@ -26,21 +26,24 @@ def test_single_pass_equals_per_kernel_at_full_size(preset, n):
     print(f"\nfull-size case {preset} n={n}", flush=True)
     m1 = StokesModel(preset, n=n).setup()
     m2 = StokesModel(preset, n=n).setup()
-    nd = m1.p["ndim"]
-    done = 0
-    for k in (3, 1):
-        m1.iterate(k)
-        m2.iterate_per_kernel(k)
-        done += k
-        print(f"  {done} iterations, comparing", flush=True)
-        for name in STATE:
-            if not valid(nd, name):
-                continue
-            a, b = m1.get(name), m2.get(name)
-            assert np.all(np.isfinite(a)), name
-            assert np.array_equal(a, b), (name, done, float(np.max(np.abs(a - b))))
-            del a, b
-    # the state moved away from the initial condition
-    assert np.max(np.abs(m1.get("Pt"))) > 0.0
-    m1.close()
-    m2.close()
+    try:
+        nd = m1.p["ndim"]
+        done = 0
+        for k in (3, 1):
+            m1.iterate(k)
+            m2.iterate_per_kernel(k)
+            done += k
+            print(f"  {done} iterations, comparing", flush=True)
+            for name in STATE:
+                if not valid(nd, name):
+                    continue
+                a, b = m1.get(name), m2.get(name)
+                assert np.all(np.isfinite(a)), name
+                assert np.array_equal(a, b), (name, done, float(np.max(np.abs(a - b))))
+                del a, b
+        # the state moved away from the initial condition
+        assert np.max(np.abs(m1.get("Pt"))) > 0.0
+    finally:
+        # free the device memory (tens of GB at 511^3) even when a check fails
+        m1.close()
+        m2.close()
