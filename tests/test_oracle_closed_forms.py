@@ -60,6 +60,23 @@ def test_hydrostatic_body_force(perf):
     assert np.max(np.abs(o.get("Vx"))) < 1e-9 and np.max(np.abs(o.get("Vy"))) < 1e-9
 
 
+@pytest.mark.parametrize("preset", ["Stokes3D_ve_perf", "Stokes3D_ve_multixpu"])
+def test_homogeneous_viscous_stress_3d(preset):
+    # 3D viscous (ve = 0), uniform μ = 1, x-z pure shear: linear V is exact,
+    # so V stays put and τ relaxes to 2μ ε̇ with ε̇xx = -εbg, ε̇zz = +εbg and
+    # every other component 0
+    o = Oracle(preset, n=9, musi=1.0, ve=0, nt=1).setup()
+    V0 = {f: o.get(f) for f in ("Vx", "Vy", "Vz")}
+    o.iterate(4000)
+    for f, v in V0.items():
+        np.testing.assert_allclose(o.get(f), v, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(o.get("txx"), -2.0, rtol=1e-12)
+    np.testing.assert_allclose(o.get("tzz"), 2.0, rtol=1e-12)
+    for f in ("tyy", "txy", "txz", "tyz"):
+        assert np.max(np.abs(o.get(f))) < 1e-12, f
+    assert np.max(np.abs(o.get("Pt"))) < 1e-10
+
+
 def test_hydrostatic_body_force_3d():
     c = 0.5
     o = Oracle("Stokes3D_ve_perf", n=9, ebg=0.0, rhog_in=c, rhog_out=c, ve=0).setup()
