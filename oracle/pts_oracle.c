@@ -20,7 +20,8 @@
  *     (output/out_Stokes2D_ve.txt) and 3D VE runs (output/out_Stokes3D_ve.txt);
  *   - closed forms: homogeneous viscosity (exact linear V, tau = 2 mu edot and
  *     the backward-Euler Maxwell recursion for VE), x<->y antisymmetry of P in
- *     pure shear, and a sparse direct solve of the steady discrete equations.
+ *     pure shear, and a direct solve of the steady discrete equations (2D, and
+ *     3D viscous).
  *   The body-force term (Stokes2D_param.jl:59) and the max-norm stop criterion
  *   have no reference fixture: the body force is pinned by the direct solve and
  *   the hydrostatic closed form, and both err forms (scaled RMS and max-norm)

@@ -168,3 +168,88 @@ def test_pt_fixed_point_equals_direct_solve(rhog_in):
     assert np.max(np.abs(o.get("Vx")[1:n, 1:n - 1] - Vx_d)) <= 1e-9 * scale_v
     assert np.max(np.abs(o.get("Vy")[1:n - 1, 1:n] - Vy_d)) <= 1e-9 * scale_v
     assert np.max(np.abs((Pt - P_d) * inner)) <= 1e-8 * np.max(np.abs(P_d))
+
+
+def _steady_residual_3d(u, n, d, Mus, rg, Vb):
+    """Steady viscous Stokes residuals on the 3D staggered grid (n³ cells,
+    spacing d): interior velocities from u, Dirichlet normal faces from Vb,
+    free-slip tangential copies, τ = 2μ ε̇ with 4-cell averages of μ for the
+    shear components, ρg_z averaged onto the Vz faces."""
+    Vx, Vy, Vz = (v.copy() for v in Vb)
+    sx, sy, sz = (n - 1, n - 2, n - 2), (n - 2, n - 1, n - 2), (n - 2, n - 2, n - 1)
+    nv = int(np.prod(sx))
+    Vx[1:n, 1:n - 1, 1:n - 1] = u[:nv].reshape(sx, order="F")
+    Vy[1:n - 1, 1:n, 1:n - 1] = u[nv:2 * nv].reshape(sy, order="F")
+    Vz[1:n - 1, 1:n - 1, 1:n] = u[2 * nv:3 * nv].reshape(sz, order="F")
+    P = u[3 * nv:].reshape((n, n, n), order="F")
+    # free-slip: copy the first interior plane outward (tangential components)
+    Vy[0], Vy[-1] = Vy[1], Vy[-2]
+    Vz[0], Vz[-1] = Vz[1], Vz[-2]
+    Vx[:, 0], Vx[:, -1] = Vx[:, 1], Vx[:, -2]
+    Vz[:, 0], Vz[:, -1] = Vz[:, 1], Vz[:, -2]
+    Vx[:, :, 0], Vx[:, :, -1] = Vx[:, :, 1], Vx[:, :, -2]
+    Vy[:, :, 0], Vy[:, :, -1] = Vy[:, :, 1], Vy[:, :, -2]
+    exx, eyy, ezz = np.diff(Vx, axis=0) / d, np.diff(Vy, axis=1) / d, np.diff(Vz, axis=2) / d
+    txx = (2 * Mus * exx)[:, 1:-1, 1:-1]
+    tyy = (2 * Mus * eyy)[1:-1, :, 1:-1]
+    tzz = (2 * Mus * ezz)[1:-1, 1:-1, :]
+
+    def avg4(a, ax1, ax2):
+        lo1, hi1 = [slice(None)] * 3, [slice(None)] * 3
+        lo1[ax1], hi1[ax1] = slice(None, -1), slice(1, None)
+        b = a[tuple(lo1)] + a[tuple(hi1)]
+        lo2, hi2 = [slice(None)] * 3, [slice(None)] * 3
+        lo2[ax2], hi2[ax2] = slice(None, -1), slice(1, None)
+        return 0.25 * (b[tuple(lo2)] + b[tuple(hi2)])
+
+    exy = 0.5 * (np.diff(Vx[1:n], axis=1) / d + np.diff(Vy[:, 1:n], axis=0) / d)
+    exz = 0.5 * (np.diff(Vx[1:n], axis=2) / d + np.diff(Vz[:, :, 1:n], axis=0) / d)
+    eyz = 0.5 * (np.diff(Vy[:, 1:n], axis=2) / d + np.diff(Vz[:, :, 1:n], axis=1) / d)
+    txy = (2 * avg4(Mus, 0, 1) * exy)[:, :, 1:-1]
+    txz = (2 * avg4(Mus, 0, 2) * exz)[:, 1:-1, :]
+    tyz = (2 * avg4(Mus, 1, 2) * eyz)[1:-1, :, :]
+    Rx = (np.diff(txx, axis=0) + np.diff(txy, axis=1) + np.diff(txz, axis=2)
+          - np.diff(P[:, 1:-1, 1:-1], axis=0)) / d
+    Ry = (np.diff(tyy, axis=1) + np.diff(txy, axis=0) + np.diff(tyz, axis=2)
+          - np.diff(P[1:-1, :, 1:-1], axis=1)) / d
+    Rz = ((np.diff(tzz, axis=2) + np.diff(txz, axis=0) + np.diff(tyz, axis=1)
+           - np.diff(P[1:-1, 1:-1, :], axis=2)) / d - 0.5 * (rg[1:-1, 1:-1, :-1] + rg[1:-1, 1:-1, 1:]))
+    return np.concatenate([Rx.ravel(order="F"), Ry.ravel(order="F"), Rz.ravel(order="F"),
+                           (exx + eyy + ezz).ravel(order="F")])
+
+
+@pytest.mark.parametrize("preset", ["Stokes3D_ve_multixpu", "Stokes3D_ve_perf"])
+@pytest.mark.parametrize("rhog_in", [0.0, -1.0])
+def test_pt_fixed_point_equals_direct_solve_3d(preset, rhog_in):
+    n = 7
+    o = Oracle(preset, n=n, ve=0, nt=1, eps=1e-12, nout=50, iter_max=400000, rhog_in=rhog_in).setup()
+    Mus, rg = o.get("Mus"), o.get("rhog")
+    Vb = (o.get("Vx"), o.get("Vy"), o.get("Vz"))
+    res = o.solve()
+    assert res["rc"] == 0 and res["err"] <= 1e-12
+    d = 10.0 / n
+    nv = (n - 1) * (n - 2) ** 2
+    N = 3 * nv + n ** 3
+    f0 = _steady_residual_3d(np.zeros(N), n, d, Mus, rg, Vb)
+    A = np.empty((N, N))
+    for k in range(N):
+        e = np.zeros(N)
+        e[k] = 1.0
+        A[:, k] = _steady_residual_3d(e, n, d, Mus, rg, Vb) - f0
+    # Pressures on box edges and corners (two or more boundary indices) appear
+    # in no momentum equation, and P is defined up to a constant: fix the sum
+    # over the other cells to the PT one and compare on them.
+    idx = np.indices((n, n, n))
+    nbound = sum(((i == 0) | (i == n - 1)).astype(int) for i in idx)
+    inner = (nbound <= 1).astype(float)
+    Pt = o.get("Pt")
+    Ag = np.vstack([A, np.concatenate([np.zeros(3 * nv), inner.ravel(order="F")])])
+    bg = np.concatenate([-f0, [(Pt * inner).sum()]])
+    u = np.linalg.lstsq(Ag, bg, rcond=None)[0]
+    scale_v = np.max(np.abs(Vb[0]))
+    got = (o.get("Vx")[1:n, 1:n - 1, 1:n - 1], o.get("Vy")[1:n - 1, 1:n, 1:n - 1], o.get("Vz")[1:n - 1, 1:n - 1, 1:n])
+    for c, g in enumerate(got):
+        want = u[c * nv:(c + 1) * nv].reshape(g.shape, order="F")
+        assert np.max(np.abs(g - want)) <= 1e-9 * scale_v, c
+    P_d = u[3 * nv:].reshape((n, n, n), order="F")
+    assert np.max(np.abs((Pt - P_d) * inner)) <= 1e-8 * np.max(np.abs(P_d))
