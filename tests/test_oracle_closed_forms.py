@@ -170,11 +170,13 @@ def test_pt_fixed_point_equals_direct_solve(rhog_in):
     assert np.max(np.abs((Pt - P_d) * inner)) <= 1e-8 * np.max(np.abs(P_d))
 
 
-def _steady_residual_3d(u, n, d, Mus, rg, Vb):
+def _steady_residual_3d(u, n, d, Mus, rg, Vb, tau_o=None, Gdt=None):
     """Steady viscous Stokes residuals on the 3D staggered grid (n³ cells,
     spacing d): interior velocities from u, Dirichlet normal faces from Vb,
     free-slip tangential copies, τ = 2μ ε̇ with 4-cell averages of μ for the
-    shear components, ρg_z averaged onto the Vz faces."""
+    shear components, ρg_z averaged onto the Vz faces.  With old stresses
+    tau_o (Maxwell, backward Euler over G·dt = Gdt) each component is
+    τ = (2ε̇ + τ_o/Gdt) / (1/μ + 1/Gdt) instead."""
     Vx, Vy, Vz = (v.copy() for v in Vb)
     sx, sy, sz = (n - 1, n - 2, n - 2), (n - 2, n - 1, n - 2), (n - 2, n - 2, n - 1)
     nv = int(np.prod(sx))
@@ -190,9 +192,14 @@ def _steady_residual_3d(u, n, d, Mus, rg, Vb):
     Vx[:, :, 0], Vx[:, :, -1] = Vx[:, :, 1], Vx[:, :, -2]
     Vy[:, :, 0], Vy[:, :, -1] = Vy[:, :, 1], Vy[:, :, -2]
     exx, eyy, ezz = np.diff(Vx, axis=0) / d, np.diff(Vy, axis=1) / d, np.diff(Vz, axis=2) / d
-    txx = (2 * Mus * exx)[:, 1:-1, 1:-1]
-    tyy = (2 * Mus * eyy)[1:-1, :, 1:-1]
-    tzz = (2 * Mus * ezz)[1:-1, 1:-1, :]
+    def law(mu, e, name):
+        if tau_o is None:
+            return 2 * mu * e
+        return (2 * e + tau_o[name] / Gdt) / (1 / mu + 1 / Gdt)
+
+    txx = law(Mus[:, 1:-1, 1:-1], exx[:, 1:-1, 1:-1], "txx")
+    tyy = law(Mus[1:-1, :, 1:-1], eyy[1:-1, :, 1:-1], "tyy")
+    tzz = law(Mus[1:-1, 1:-1, :], ezz[1:-1, 1:-1, :], "tzz")
 
     def avg4(a, ax1, ax2):
         lo1, hi1 = [slice(None)] * 3, [slice(None)] * 3
@@ -205,9 +212,9 @@ def _steady_residual_3d(u, n, d, Mus, rg, Vb):
     exy = 0.5 * (np.diff(Vx[1:n], axis=1) / d + np.diff(Vy[:, 1:n], axis=0) / d)
     exz = 0.5 * (np.diff(Vx[1:n], axis=2) / d + np.diff(Vz[:, :, 1:n], axis=0) / d)
     eyz = 0.5 * (np.diff(Vy[:, 1:n], axis=2) / d + np.diff(Vz[:, :, 1:n], axis=1) / d)
-    txy = (2 * avg4(Mus, 0, 1) * exy)[:, :, 1:-1]
-    txz = (2 * avg4(Mus, 0, 2) * exz)[:, 1:-1, :]
-    tyz = (2 * avg4(Mus, 1, 2) * eyz)[1:-1, :, :]
+    txy = law(avg4(Mus, 0, 1)[:, :, 1:-1], exy[:, :, 1:-1], "txy")
+    txz = law(avg4(Mus, 0, 2)[:, 1:-1, :], exz[:, 1:-1, :], "txz")
+    tyz = law(avg4(Mus, 1, 2)[1:-1, :, :], eyz[1:-1, :, :], "tyz")
     Rx = (np.diff(txx, axis=0) + np.diff(txy, axis=1) + np.diff(txz, axis=2)
           - np.diff(P[:, 1:-1, 1:-1], axis=0)) / d
     Ry = (np.diff(tyy, axis=1) + np.diff(txy, axis=0) + np.diff(tyz, axis=2)
@@ -219,23 +226,31 @@ def _steady_residual_3d(u, n, d, Mus, rg, Vb):
 
 
 @pytest.mark.parametrize("preset", ["Stokes3D_ve_multixpu", "Stokes3D_ve_perf"])
-@pytest.mark.parametrize("rhog_in", [0.0, -1.0])
-def test_pt_fixed_point_equals_direct_solve_3d(preset, rhog_in):
+@pytest.mark.parametrize("rhog_in,ve", [(0.0, 0), (-1.0, 0), (0.0, 1)])
+def test_pt_fixed_point_equals_direct_solve_3d(preset, rhog_in, ve):
     n = 7
-    o = Oracle(preset, n=n, ve=0, nt=1, eps=1e-12, nout=50, iter_max=400000, rhog_in=rhog_in).setup()
+    o = Oracle(preset, n=n, ve=ve, nt=1, eps=1e-12, nout=50, iter_max=400000, rhog_in=rhog_in).setup()
     Mus, rg = o.get("Mus"), o.get("rhog")
     Vb = (o.get("Vx"), o.get("Vy"), o.get("Vz"))
+    tau_o, Gdt = None, None
+    if ve:
+        # random old stresses: the solve's assign_τ! copies τ into τ_o
+        rng = np.random.default_rng(3)
+        tau_o = {f: rng.uniform(-1.0, 1.0, o.shape(f)) for f in ("txx", "tyy", "tzz", "txy", "txz", "tyz")}
+        for f, v in tau_o.items():
+            o.set(f, v)
+        Gdt = o.p["G"] * o.p["dt"]
     res = o.solve()
     assert res["rc"] == 0 and res["err"] <= 1e-12
     d = 10.0 / n
     nv = (n - 1) * (n - 2) ** 2
     N = 3 * nv + n ** 3
-    f0 = _steady_residual_3d(np.zeros(N), n, d, Mus, rg, Vb)
+    f0 = _steady_residual_3d(np.zeros(N), n, d, Mus, rg, Vb, tau_o, Gdt)
     A = np.empty((N, N))
     for k in range(N):
         e = np.zeros(N)
         e[k] = 1.0
-        A[:, k] = _steady_residual_3d(e, n, d, Mus, rg, Vb) - f0
+        A[:, k] = _steady_residual_3d(e, n, d, Mus, rg, Vb, tau_o, Gdt) - f0
     # Pressures on box edges and corners (two or more boundary indices) appear
     # in no momentum equation, and P is defined up to a constant: fix the sum
     # over the other cells to the PT one and compare on them.
