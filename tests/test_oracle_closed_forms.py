@@ -115,7 +115,7 @@ def test_pure_shear_antisymmetry_3d(preset):
 # free-slip tangential copies, Dirichlet normal velocities): the PT fixed
 # point must be that solution.  Pressure is fixed up to a constant.
 # ---------------------------------------------------------------------------
-def _steady_residual(u, nx, ny, dx, dy, Mus, rg, Vx_b, Vy_b):
+def _steady_residual(u, nx, ny, dx, dy, Mus, rg, Vx_b, Vy_b, tau_o=None, Gdt=None):
     nvx, nvy = (nx - 1) * (ny - 2), (nx - 2) * (ny - 1)
     Vx = Vx_b.copy()
     Vy = Vy_b.copy()
@@ -126,32 +126,47 @@ def _steady_residual(u, nx, ny, dx, dy, Mus, rg, Vx_b, Vy_b):
     Vy[0, :], Vy[-1, :] = Vy[1, :], Vy[-2, :]
     exx = np.diff(Vx, axis=0) / dx
     eyy = np.diff(Vy, axis=1) / dy
-    txx, tyy = 2 * Mus * exx, 2 * Mus * eyy
+    def law(mu, e, name):
+        # viscous, or Maxwell with old stress τ_o over G·dt (backward Euler)
+        if tau_o is None:
+            return 2 * mu * e
+        return (2 * e + tau_o[name] / Gdt) / (1 / mu + 1 / Gdt)
+
+    txx, tyy = law(Mus, exx, "txx"), law(Mus, eyy, "tyy")
     mv = 0.25 * (Mus[:-1, :-1] + Mus[1:, :-1] + Mus[:-1, 1:] + Mus[1:, 1:])
     exy = 0.5 * (np.diff(Vx[1:nx, :], axis=1) / dy + np.diff(Vy[:, 1:ny], axis=0) / dx)
-    txy = 2 * mv * exy
+    txy = law(mv, exy, "txy")
     Rx = np.diff(txx[:, 1:-1], axis=0) / dx + np.diff(txy[:, :], axis=1)[:, :] / dy - np.diff(P[:, 1:-1], axis=0) / dx
     Ry = (np.diff(tyy[1:-1, :], axis=1) / dy + np.diff(txy, axis=0) / dx - np.diff(P[1:-1, :], axis=1) / dy
           - 0.5 * (rg[1:-1, :-1] + rg[1:-1, 1:]))
     return np.concatenate([Rx.ravel(order="F"), Ry.ravel(order="F"), (exx + eyy).ravel(order="F")])
 
 
-@pytest.mark.parametrize("rhog_in", [0.0, -1.0])
-def test_pt_fixed_point_equals_direct_solve(rhog_in):
+@pytest.mark.parametrize("preset,rhog_in", [("Stokes2D", 0.0), ("Stokes2D", -1.0), ("Stokes2D_ve", 0.0),
+                                            ("Stokes2D_ve_perf", 0.0)])
+def test_pt_fixed_point_equals_direct_solve(preset, rhog_in):
     n = 13
-    o = Oracle("Stokes2D", n=n, eps=1e-12, nout=50, iter_max=200000, rhog_in=rhog_in).setup()
+    o = Oracle(preset, n=n, nt=1, eps=1e-12, nout=50, iter_max=200000, rhog_in=rhog_in).setup()
     Mus, rg, Vx_b, Vy_b = o.get("Mus"), o.get("rhog"), o.get("Vx"), o.get("Vy")
+    tau_o, Gdt = None, None
+    if o.p["ve"]:
+        # random old stresses: the solve's assign_τ! copies τ into τ_o
+        rng = np.random.default_rng(5)
+        tau_o = {f: rng.uniform(-1.0, 1.0, o.shape(f)) for f in ("txx", "tyy", "txy")}
+        for f, v in tau_o.items():
+            o.set(f, v)
+        Gdt = o.p["G"] * o.p["dt"]
     res = o.solve()
     assert res["rc"] == 0 and res["err"] <= 1e-12
     dx = dy = 10.0 / n
     nvx, nvy, npp = (n - 1) * (n - 2), (n - 2) * (n - 1), n * n
     N = nvx + nvy + npp
-    f0 = _steady_residual(np.zeros(N), n, n, dx, dy, Mus, rg, Vx_b, Vy_b)
+    f0 = _steady_residual(np.zeros(N), n, n, dx, dy, Mus, rg, Vx_b, Vy_b, tau_o, Gdt)
     A = np.empty((N, N))
     for k in range(N):
         e = np.zeros(N)
         e[k] = 1.0
-        A[:, k] = _steady_residual(e, n, n, dx, dy, Mus, rg, Vx_b, Vy_b) - f0
+        A[:, k] = _steady_residual(e, n, n, dx, dy, Mus, rg, Vx_b, Vy_b, tau_o, Gdt) - f0
     # The four corner pressures appear in no momentum equation (free), and P is
     # defined up to a constant: fix the sum over the other cells to the PT one
     # and compare away from the corners.
