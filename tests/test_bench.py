@@ -28,3 +28,62 @@ def test_traffic_table_matches_the_counter_sums():
     with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
         assert set(json.load(f)) >= {"k2_iter_8191", "k3_iter2_511"}
     assert bench.load_traffic("no_such_kernel") is None
+
+
+def test_inversion_gradient_leg_drives_chunked_forward_solves(monkeypatch):
+    # bench_c5_grad's control flow with a stand-in for the device inversion:
+    # forward solves are driven in check-aligned chunks until converged and
+    # every phase lands in the record
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    import bench
+    import ptstokes.adjoint as adjoint
+
+    calls = []
+
+    class FakeCtx:
+        def __init__(self):
+            self.calls = calls
+
+        def pt_solve(self, params, eps, nout, iter_max, norm_kind):
+            self.calls.append((eps, nout, iter_max, norm_kind))
+            done = len(self.calls) % 3 == 0
+            return dict(iters=500 if done else iter_max + 1, err=1e-11 if done else 1e-3, converged=done)
+
+        def sync(self):
+            pass
+
+    class FakeInversion:
+        def __init__(self, n, npts, eps, adj_eps):
+            assert (npts, eps, adj_eps) == (256, 1e-10, 1e-10)
+            self.ctx, self.params, self.eps, self.nout, self.iter_max = FakeCtx(), None, eps, 200, 10 ** 7
+            self.model = type("M", (), {"p": dict(norm_kind=1)})()
+            self.last, self.obs = {}, None
+
+        def grf_logmu(self, seed):
+            return np.zeros((n, n))
+
+        def set_logmu(self, m):
+            pass
+
+        def top_velocity(self):
+            return np.ones(256)
+
+        def dJ_dVx(self):
+            return None, 0.25
+
+        def gradient(self):
+            self.last["adjoint"] = dict(iters=42, err=1e-11)
+            return np.ones((n, n))
+
+        def close(self):
+            pass
+
+    n = 15
+    monkeypatch.setattr(adjoint, "Inversion", FakeInversion)
+    out = bench.bench_c5_grad(n, chunk=1000)
+    assert out["observe_iters"] == out["forward_iters"] == 2 * 1000 + 500
+    assert out["adjoint_iters"] == 42 and out["J"] == 0.25 and out["grad_finite"]
+    assert out["forward_err"] == 1e-11
+    # each chunk runs exactly `chunk` iterations at the inversion's tolerance and check interval
+    assert len(calls) == 6 and all(c == (1e-10, 200, 999, 1) for c in calls)
