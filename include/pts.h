@@ -84,11 +84,16 @@ int pts_fill(pts_ctx* ctx, int field, double value);
 int pts_copy(pts_ctx* ctx, int dst, int src);
 /* Swap two fields of equal shape ("Mus, Mus2 = Mus2, Mus", Stokes2D_ve.jl:140). */
 int pts_swap(pts_ctx* ctx, int a, int b);
+/* Wait for every call enqueued on this context.  No separate call in the
+ * reference scripts: there a kernel's results are complete when @parallel
+ * returns; here this is the point where the caller may rely on that. */
 int pts_sync(pts_ctx* ctx);
-/* The HIP stream (hipStream_t) every kernel of this context is enqueued on,
- * so a caller can record events or order its own work against it. */
+/* No counterpart in the reference.  The HIP stream (hipStream_t) every kernel
+ * of this context is enqueued on, so a caller can record events or order its
+ * own work against it. */
 int pts_get_stream(const pts_ctx* ctx, void** stream);
-/* Bytes of device memory held by the context. */
+/* No counterpart in the reference.  Bytes of device memory held by the
+ * context. */
 int pts_device_bytes(const pts_ctx* ctx, int64_t* bytes);
 /* No counterpart in the reference (placement of this library's device
  * blocks).  On its first pts_pt_iterate / pts_pt_solve, a single-pass context
