@@ -70,6 +70,10 @@ typedef struct pts_grid {
 typedef struct pts_ctx pts_ctx;
 
 /* ---- lifecycle and fields ------------------------------------------------ */
+/* A context owns the arrays each script allocates with @zeros and
+ * Data.Array (Stokes2D_ve.jl:111-136, Stokes3D_ve_perf.jl:141-172), addressed
+ * by field id with the same shapes; fields are allocated on first use.
+ * pts_last_error(NULL) reports a failed pts_create. */
 int pts_create(const pts_grid* grid, pts_ctx** out);
 int pts_destroy(pts_ctx* ctx);
 const char* pts_last_error(const pts_ctx* ctx);
