@@ -111,6 +111,14 @@ int pts_device_bytes(const pts_ctx* ctx, int64_t* bytes);
  * original blocks, ms[t] try t, 0 for sets not tried (at most 32). */
 int pts_placement_times(const pts_ctx* ctx, float* ms, int32_t n);
 
+/* No counterpart in the reference.  Sets one launch-configuration knob of
+ * the single-pass kernels by the name of the environment variable that sets
+ * it at pts_create (PTS_PLANES_PER_BLOCK, PTS_3D_VARIANT, PTS_3D_MAP_FLAT,
+ * PTS_ROWS_PER_BLOCK, PTS_2D_VARIANT, PTS_2D_PERSIST), so
+ * that configurations can be compared on one set of device blocks.  Results
+ * are bit-identical for every value; PTS_ERR_ARG for an unknown name. */
+int pts_set_knob(pts_ctx* ctx, const char* name, int64_t value);
+
 /* ---- setup kernels ------------------------------------------------------- */
 /* Initial conditions of every reference script (Stokes2D_ve.jl:131-136,
  * Stokes3D_ve_perf.jl:167-172; ic_igg=1 uses ImplicitGlobalGrid's x_g()

@@ -1104,6 +1104,23 @@ static int place_fields(pts_ctx* c, const StepScalars& s) {
     return rc;
 }
 
+int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
+    if (!c || !name) return fail(c, PTS_ERR_ARG, "null argument");
+    const std::string k(name);
+    const int v = (int)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, value));
+    if (k == "PTS_PLANES_PER_BLOCK") c->planes_per_block = v;
+    else if (k == "PTS_3D_VARIANT") c->variant_3d = v;
+    else if (k == "PTS_3D_MAP_FLAT") c->map_flat_3d = v != 0;
+    else if (k == "PTS_ROWS_PER_BLOCK") c->rows_per_block = v;
+    else if (k == "PTS_2D_VARIANT") c->variant_2d = v;
+    else if (k == "PTS_2D_PERSIST") c->persist_2d = v != 0;
+    else return fail(c, PTS_ERR_ARG, "unknown knob " + k);
+    // captured graphs hold the old launch configuration
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    c->graphs.clear();
+    return PTS_OK;
+}
+
 int pts_placement_times(const pts_ctx* c, float* ms, int32_t n) {
     if (!c || !ms || n < 0) return PTS_ERR_ARG;
     for (int q = 0; q < n; ++q) ms[q] = q < 32 ? c->place_ms[q] : 0.f;

@@ -472,7 +472,7 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
     __shared__ double shA[2][4][BY][64];
     // next-plane inputs: Gdτ, dτ_Rho, μ (MX: μ of the plane after), Vx, Vz (row +1), Vy (row -1); Vz, μ (start)
     __shared__ double shB[8][BY][64];
-    static_assert(!(MX && (PF || !DP)), "MX: derived parameters, loads after the arithmetic");
+    static_assert(!(MX && !DP), "MX: derived parameters");
 
     // Block b runs on XCD b % 8.  Each XCD owns a contiguous eighth of the
     // tiles (neighbouring tiles share halo rows in its L2) and takes the
@@ -564,8 +564,16 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
     auto load = [&](auto role, int k, Plane3& I, bool start) __attribute__((always_inline)) {
         constexpr int R = decltype(role)::value;
         const uint32_t t = tko(k), ti = t + oin, to = t + oold;
-        if (mt_loaded(role, k, start)) I.g = ldb(Gp, t);
-        else if (lane == 0 || lane == 63) I.g = ldb(Gp, t);  // MX: the wave-edge lanes lack an x-neighbour
+        if (mt_loaded(role, k, start)) {
+            I.g = ldb(Gp, t);
+        } else if (lane == 0 || lane == 63) {
+            // MX: μ of the x-neighbour outside the wave (c - 1, c + 1, clamped
+            // like tb), held where the loaded Musτ would be
+            int cl = c;
+            asm volatile("" : "+v"(cl));
+            const int cc = min(max(cl, 0), nx), cf = lane == 0 ? max(cl - 1, 0) : min(cl + 1, nx);
+            I.g = ldb(Mup, t + (uint32_t)((cf - cc) * 8));
+        }
         if constexpr (!DP && R != 2) I.h = ldb(Rhp, t);
         if constexpr (MX) {
             I.mun = ldb(Mup, t + sxy8);  // μ(k) comes from the previous plane's load
@@ -608,15 +616,20 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
                 if (!mt_loaded(role, k, start)) {
                     // cross maximum in compute_maxloc!'s order: x, centre, y, z;
                     // the row neighbours came one plane ahead, μ(k-1) is carried.
-                    // Lanes 0 and 63 (no x-neighbour in the wave) keep the loaded
-                    // Musτ; their Gdτ feeds the edges of lanes 1 and 62.
-                    double m = fmax(from_left(I.mu), from_right(I.mu));
+                    // Lanes 0 and 63 take the x-neighbour outside the wave from
+                    // their own load (their Gdτ feeds the edges of lanes 1 and
+                    // 62); a maximum is exact, so the order of the terms does
+                    // not change the result.
+                    double xl = from_left(I.mu), xr = from_right(I.mu);
+                    if (lane == 0) xl = I.g;
+                    if (lane == 63) xr = I.g;
+                    double m = fmax(xl, xr);
                     m = fmax(m, I.mu);
                     m = fmax(m, U.muUn);
                     m = fmax(m, U.muDn);
                     m = fmax(m, P.muP);
                     m = fmax(m, I.mun);
-                    const double m1 = (lane == 0 || lane == 63) ? I.g : m;
+                    const double m1 = m;
                     // free-slip copies (bc_x!): the boundary columns take their neighbour's
                     const double mr = from_right(m1), ml = from_left(m1);
                     M = c == 0 ? mr : (c == nx - 1 ? ml : m1);
@@ -875,6 +888,11 @@ template <bool VE, bool MUL, bool BF, bool DP>
 void go3b_pf(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bool pf, bool g2, bool mx,
              bool map_flat) {
     if constexpr (DP) {
+        if (mx && pf) {
+            if (g2) go3b<VE, MUL, BF, DP, true, true, true>(s, in, c, Lz, map_flat);
+            else go3b<VE, MUL, BF, DP, true, false, true>(s, in, c, Lz, map_flat);
+            return;
+        }
         if (mx && !pf) {
             if (g2) go3b<VE, MUL, BF, DP, false, true, true>(s, in, c, Lz, map_flat);
             else go3b<VE, MUL, BF, DP, false, false, true>(s, in, c, Lz, map_flat);

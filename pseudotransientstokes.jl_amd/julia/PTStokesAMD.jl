@@ -87,6 +87,9 @@ function placement_times(c::Ctx, n=32)
     chk(c, ccall((:pts_placement_times, libpts), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int32), c.h, ms, n))
     return ms
 end
+# launch-configuration knob by its environment-variable name (no counterpart in the reference)
+set_knob!(c::Ctx, name::AbstractString, value::Integer) =
+    chk(c, ccall((:pts_set_knob, libpts), Cint, (Ptr{Cvoid}, Cstring, Int64), c.h, name, value))
 
 # setup kernels
 init_inclusion!(c::Ctx, lx, ly, lz, μs0, μsi, rad2, εbg, ρg_in=0.0, ρg_out=0.0; igg=false) =

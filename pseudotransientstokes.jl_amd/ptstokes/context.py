@@ -110,6 +110,11 @@ class Context:
         self._chk(self.lib.pts_placement_times(self.h, ms, 32))
         return [round(float(x), 4) for x in ms if x > 0]
 
+    def set_knob(self, name, value):
+        """Launch-configuration knob of the single-pass kernels (the name of
+        the environment variable that sets it at creation)."""
+        self._chk(self.lib.pts_set_knob(self.h, name.encode(), int(value)))
+
     # -- setup kernels -------------------------------------------------------
     def init_inclusion(self, lx, ly, lz=10.0, mus0=1.0, musi=1e-3, rad2=1.0, ebg=1.0, rhog_in=0.0,
                        rhog_out=0.0, igg=False):

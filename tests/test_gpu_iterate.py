@@ -178,11 +178,16 @@ def test_moved_blocks_keep_state_and_graphs(accept, dim, monkeypatch):
                                        ("Stokes2D_ve_perf", dict(nx=1100, ny=75)),
                                        ("Stokes2D", dict(nx=600, ny=130, rhog_in=-1.0)),
                                        ("Stokes2D_ve", dict(nx=509, ny=67, dt=0.1))])
-def test_formed_mustau_rough_viscosity(preset, kw):
+@pytest.mark.parametrize("variant", ["1", "2"])
+def test_formed_mustau_rough_viscosity(preset, kw, variant, monkeypatch):
     # A rough μ (every cross maximum may sit at any neighbour, also across the
     # 62-column tile edges) with Musτ = maxloc(μ) + the free-slip copies, so
     # that the kernels form Musτ from μ; the oracle reads the same Musτ, Gdτ
-    # and dτ_Rho.
+    # and dτ_Rho.  3D: without (1) and with (2) the next plane's loads issued
+    # before the arithmetic.
+    if preset.startswith("Stokes2D") and variant == "2":
+        pytest.skip("3D kernel variant")
+    monkeypatch.setenv("PTS_3D_VARIANT", variant)
     m, o = pair(preset, **kw)
     rng = np.random.default_rng(11)
     mu = np.exp(rng.normal(0.0, 1.5, size=o.shape("Mus")))
