@@ -20,14 +20,18 @@ from ptstokes import StokesModel
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("preset,n", [("Stokes2D_ve_perf", 8191), ("Stokes3D_ve_perf", 511)],
-                         ids=["2d-ve-8191", "3d-ve-511"])
-def test_single_pass_equals_per_kernel_at_full_size(preset, n):
-    print(f"\nfull-size case {preset} n={n}", flush=True)
-    m1 = StokesModel(preset, n=n).setup()
-    m2 = StokesModel(preset, n=n).setup()
+# 2D VE 8191² with the benchmark's dt = η/G/10 = 0.1; 3D VE 511³ with the
+# script's dt = 1 (Stokes3D_ve_perf.jl:124-126)
+@pytest.mark.parametrize("preset,n,kw", [("Stokes2D_ve_perf", 8191, dict(dt=0.1)), ("Stokes3D_ve_perf", 511, {})],
+                         ids=["2d-ve-8191-dt0.1", "3d-ve-511"])
+def test_single_pass_equals_per_kernel_at_full_size(preset, n, kw):
+    print(f"\nfull-size case {preset} n={n} {kw}", flush=True)
+    m1 = StokesModel(preset, n=n, **kw).setup()
+    m2 = StokesModel(preset, n=n, **kw).setup()
     try:
         nd = m1.p["ndim"]
+        m1.ctx.assign_tau()  # the physical step starts with τ_o = τ (Stokes2D_ve_perf.jl:146)
+        m2.ctx.assign_tau()
         done = 0
         for k in (3, 1):
             m1.iterate(k)

@@ -1,5 +1,7 @@
 """Per-kernel parity: every reference kernel on the GPU against the oracle,
 bit for bit, on the reference set-up and on seeded random states."""
+import zlib
+
 import numpy as np
 import pytest
 
@@ -62,7 +64,7 @@ def _run_gpu(m, name):
 @pytest.mark.parametrize("kernel", DIV_KERNELS + MUL_KERNELS)
 def test_kernel_bit_exact_on_random_state(preset, kw, kernel):
     m, o = pair(preset, **kw)
-    randomise(m, o, seed=hash((preset, kernel)) % 2**31)
+    randomise(m, o, seed=zlib.crc32(f"{preset}/{kernel}".encode()))
     nd = m.p["ndim"]
     if kernel == "compute_V":  # consumes dV written by compute_dV
         _run_gpu(m, "compute_dV")

@@ -124,3 +124,24 @@ def test_chunked_solve_equals_one_loop(preset, kw):
         assert np.array_equal(a.get(f), b.get(f)), f
     a.close()
     b.close()
+
+
+def test_c1_viscous_127_dispersion_optimum_bit_exact():
+    # 2D viscous Stokes (Stokes2D.jl, divide form) at 127^2: 1e-3 inclusion of
+    # radius 1 (10 smoothing passes) with rho_g_y = -1 inside and 0 outside
+    # (Stokes2D_param.jl:59), free-slip walls without background shear,
+    # Re = 9*sqrt(3)*pi/4 and r = 1/4 (the 1D dispersion optimum,
+    # dispersion_analysis_stokes1D.m:34-35), stopped when the scaled max-norm
+    # of the residuals is below 1e-8, checked every 200 iterations.  CFL is
+    # the script's 0.9/sqrt(2) (Stokes2D.jl:82): 0.95/sqrt(2) diverges on this
+    # problem (test_oracle_closed_forms.py::test_c1_cfl_095_diverges).
+    from ptstokes.models import R_OPT, RE_OPT
+    kw = dict(n=127, CFL=0.9 / np.sqrt(2.0), Re=RE_OPT, r=R_OPT, rhog_in=-1.0, ebg=0.0, norm_kind=1, nout=200,
+              eps=1e-8)
+    m, o = pair("Stokes2D", **kw)
+    r = m.step()
+    res = o.solve()
+    assert res["rc"] == 0 and r["converged"]
+    assert r["iters"] == res["ittot"] == 10200
+    assert compare(m, o, STATE) == []
+    m.close()

@@ -283,3 +283,17 @@ def test_pt_fixed_point_equals_direct_solve_3d(preset, rhog_in, ve):
         assert np.max(np.abs(g - want)) <= 1e-9 * scale_v, c
     P_d = u[3 * nv:].reshape((n, n, n), order="F")
     assert np.max(np.abs((Pt - P_d) * inner)) <= 1e-8 * np.max(np.abs(P_d))
+
+
+def test_c1_cfl_095_diverges():
+    # The 127^2 viscous inclusion problem with rho_g_y = -1, no background
+    # shear and the dispersion-optimal Re, r: with the CFL number 0.95/sqrt(2)
+    # the PT iteration blows up (NaN at a residual check), with the script's
+    # 0.9/sqrt(2) (Stokes2D.jl:82) it converges to the scaled max-norm 1e-8
+    # in 10200 iterations.
+    from oracle import R_OPT, RE_OPT
+    kw = dict(n=127, Re=RE_OPT, r=R_OPT, rhog_in=-1.0, ebg=0.0, norm_inf=1, nout=200, eps=1e-8)
+    bad = Oracle("Stokes2D", CFL=0.95 / np.sqrt(2.0), **kw).setup().solve()
+    assert bad["rc"] == -1 and np.isnan(bad["err"]) and bad["ittot"] < 5000, bad
+    good = Oracle("Stokes2D", CFL=0.9 / np.sqrt(2.0), **kw).setup().solve()
+    assert good["rc"] == 0 and good["err"] < 1e-8 and good["ittot"] == 10200, good
