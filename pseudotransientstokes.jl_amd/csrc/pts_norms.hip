@@ -4,9 +4,11 @@
 //
 // Two passes with a fixed grid (kRedBlocks blocks, 256 threads), so the
 // result is deterministic run to run: pass 1 writes one partial per block,
-// pass 2 folds them in one block.  "Live" mode computes R and ∇V on the fly
-// from the state (no residual arrays are written or read); "stored" mode
-// reduces the Rx/Ry/Rz/∇V arrays written by compute_Res!.
+// pass 2 folds them in one block.  Within a block each wavefront reduces by
+// cross-lane shuffles and the four wave totals meet in LDS.  "Live" mode
+// computes R and ∇V on the fly from the state (no residual arrays are written
+// or read); "stored" mode reduces the Rx/Ry/Rz/∇V arrays written by
+// compute_Res!.
 #include "pts_internal.hpp"
 
 namespace pts {
@@ -40,24 +42,36 @@ __device__ __forceinline__ void acc_merge(Acc& a, const Acc& b) {
     for (int q = 8; q < 12; ++q) a.v[q] = nanmax(a.v[q], b.v[q]);
 }
 
-__device__ void block_reduce_store(Acc& a, double* dst) {
-    __shared__ double sm[kRedVals][kThreads];
-    const int t = threadIdx.x;
-    for (int q = 0; q < kRedVals; ++q) sm[q][t] = a.v[q];
-    __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-        if (t < w) {
-            Acc b;
-            for (int q = 0; q < kRedVals; ++q) {
-                a.v[q] = sm[q][t];
-                b.v[q] = sm[q][t + w];
-            }
-            acc_merge(a, b);
-            for (int q = 0; q < kRedVals; ++q) sm[q][t] = a.v[q];
-        }
-        __syncthreads();
+// Butterfly over the 64 lanes of a wavefront (cross-lane shuffles, no LDS
+// memory): every lane ends with the wave's total.  acc_merge is symmetric in
+// its arguments, so both lanes of a pair compute the same bits.
+__device__ __forceinline__ void wave_reduce(Acc& a) {
+    for (int m = 32; m >= 1; m >>= 1) {
+        Acc b;
+        for (int q = 0; q < kRedVals; ++q) b.v[q] = __shfl_xor(a.v[q], m, 64);
+        acc_merge(a, b);
     }
-    if (t < kRedVals) dst[t] = sm[t][0];
+}
+
+// Wave totals through LDS (one barrier), folded by thread 0 in wave order.
+__device__ void block_reduce_store(Acc& a, double* dst) {
+    constexpr int kWaves = kThreads / 64;
+    __shared__ double sm[kWaves][kRedVals];
+    wave_reduce(a);
+    const int t = threadIdx.x, w = t / 64;
+    if ((t & 63) == 0)
+        for (int q = 0; q < kRedVals; ++q) sm[w][q] = a.v[q];
+    __syncthreads();
+    if (t == 0) {
+        Acc r;
+        for (int q = 0; q < kRedVals; ++q) r.v[q] = sm[0][q];
+        for (int v = 1; v < kWaves; ++v) {
+            Acc b;
+            for (int q = 0; q < kRedVals; ++q) b.v[q] = sm[v][q];
+            acc_merge(r, b);
+        }
+        for (int q = 0; q < kRedVals; ++q) dst[q] = r.v[q];
+    }
 }
 
 template <bool MUL>
