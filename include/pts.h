@@ -178,6 +178,22 @@ typedef struct pts_norms {
 } pts_norms;
 /* Reductions of the stored Rx, Ry, Rz, ∇V, Vx, Pt arrays (Stokes2D_ve.jl:164-168). */
 int pts_norms_stored(pts_ctx* ctx, pts_norms* out);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops).  The reductions of
+ * pts_norms_stored over caller-owned device arrays in the reference's dense
+ * column-major shapes (pts_field_shape of a context of this grid): Vx, Pt,
+ * Rx, Ry, Rz (3D; may be NULL in 2D), ∇V.  Enqueued on `stream` (a
+ * hipStream_t; NULL: the default stream) of the current device without
+ * synchronising; d_out (device memory, 12 doubles) receives Vmin, Vmax, Pmin,
+ * Pmax, Σ R² (Rx, Ry, Rz, ∇V) and max|R| (same order).  It calls no
+ * hipMalloc or hipFree: the partial results live in a 24 KiB file-scope
+ * device array, one per GPU, shared by every call on that GPU, so calls that
+ * overlap on one GPU race.  Arrays of more than 2^31-1 elements, a bad ndim
+ * or extent, or a null array give PTS_ERR_ARG (message in
+ * pts_last_error(NULL)). */
+int pts_norms_dense_noalloc(int32_t ndim, int32_t nx, int32_t ny, int32_t nz, const double* Vx, const double* Pt,
+                            const double* Rx, const double* Ry, const double* Rz, const double* divV, void* stream,
+                            double* d_out);
 /* err from norms: max over components of ‖R‖₂/ΔP·lx/√len (kind 0, the
  * reference's err) or max|R|/ΔP·lx (kind 1); ∇V scaled by ΔV = Vmax-Vmin. */
 double pts_err_from_norms(const pts_norms* n, int32_t ndim, double lx, int32_t kind, double comp[4]);

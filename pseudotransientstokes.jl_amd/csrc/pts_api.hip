@@ -288,6 +288,8 @@ int state_fields(const pts_ctx* c, int* out) {
 }  // namespace
 
 namespace pts {
+void set_global_error(const std::string& msg) { g_create_error = msg; }
+
 void field_shape(int ndim, int nx, int ny, int nz, int f, int64_t s[3]) {
     s[0] = nx; s[1] = ny; s[2] = ndim == 3 ? nz : 1;
     if (ndim == 2) {

@@ -237,6 +237,18 @@ hipError_t launch2_adj_change(hipStream_t s, const Fields& f, const Fields& a, c
 constexpr int kRedBlocks = 512;
 constexpr int kRedVals = 12;  // Vmin Vmax Pmin Pmax ss[4] ma[4]
 hipError_t launch_norms_stored(hipStream_t s, const Fields& f, int ndim, double* partials, double* out);
+// Reductions of launch_norms_stored over dense device arrays (reference
+// shapes): `nblocks` partial slots of kRedVals doubles in `partials`, the
+// result (kRedVals doubles) in `out`.
+struct DenseNorms {
+    const double* vx;
+    const double* pt;
+    const double* r[4];  // Rx, Ry, Rz, ∇V (null: none)
+    int64_t nvx, npt, nr[4];
+};
+hipError_t launch_norms_dense(hipStream_t s, const DenseNorms& d, double* partials, int nblocks, double* out);
+// Message reported by pts_last_error(NULL) (calls without a context).
+void set_global_error(const std::string& msg);
 // On-the-fly residual norms of the current state: R from Pt/τ, ∇V from the
 // V fields in `fv` (pass the pre-update V for the 4-kernel semantics).
 hipError_t launch_norms_live(hipStream_t s, const Fields& f, const Fields& fv, const StepScalars& c, int ndim,

@@ -178,7 +178,38 @@ __global__ __launch_bounds__(kThreads) void k_norms_final(const double* partials
     block_reduce_store(a, out);
 }
 
+// The same reductions over dense arrays (the reference's own shapes, column
+// major): min/max of Vx and Pt, Σ R² and max|R| of each residual; a null
+// residual pointer contributes nothing.
+__global__ __launch_bounds__(kThreads) void k_norms_dense(DenseNorms d, double* partials) {
+    Acc a;
+    acc_init(a);
+    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = t0; q < d.nvx; q += stride) {
+        const double v = d.vx[q];
+        a.v[0] = nanmin(a.v[0], v);
+        a.v[1] = nanmax(a.v[1], v);
+    }
+    for (int64_t q = t0; q < d.npt; q += stride) {
+        const double v = d.pt[q];
+        a.v[2] = nanmin(a.v[2], v);
+        a.v[3] = nanmax(a.v[3], v);
+    }
+    for (int c = 0; c < 4; ++c) {
+        const double* r = d.r[c];
+        if (!r) continue;
+        for (int64_t q = t0; q < d.nr[c]; q += stride) acc_res(a, c, r[q]);
+    }
+    block_reduce_store(a, partials + (int64_t)blockIdx.x * kRedVals);
+}
+
 }  // namespace
+
+hipError_t launch_norms_dense(hipStream_t s, const DenseNorms& d, double* partials, int nblocks, double* out) {
+    k_norms_dense<<<nblocks, kThreads, 0, s>>>(d, partials);
+    k_norms_final<<<1, kThreads, 0, s>>>(partials, nblocks, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_norms_stored(hipStream_t s, const Fields& f, int ndim, double* partials, double* out) {
     StepScalars c{};

@@ -146,6 +146,17 @@ function norms(c::Ctx)
     chk(c, ccall((:pts_norms_stored, libpts), Cint, (Ptr{Cvoid}, Ref{Norms}), c.h, n))
     return n[]
 end
+# The same reductions over caller-owned device arrays (reference shapes), for
+# callers that cannot allocate (no counterpart in the reference): enqueued on
+# `stream`, 12 doubles into the device buffer `out`.
+function norms_dense_noalloc!(out, Vx, Pt, Rx, Ry, Rz, ∇V, ndim, nx, ny, nz; stream=C_NULL)
+    rc = ccall((:pts_norms_dense_noalloc, libpts), Cint,
+               (Int32, Int32, Int32, Int32, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
+                Ptr{Cdouble}, Ptr{Cvoid}, Ptr{Cdouble}),
+               ndim, nx, ny, nz, Vx, Pt, Rx, Ry, Rz, ∇V, stream, out)
+    rc == 0 || error("libpts error $rc: " * unsafe_string(ccall((:pts_last_error, libpts), Cstring, (Ptr{Cvoid},), C_NULL)))
+    return out
+end
 function err_from_norms(n::Norms, ndim, lx; kind=0)
     comp = zeros(Float64, 4)
     err = ccall((:pts_err_from_norms, libpts), Float64, (Ref{Norms}, Int32, Float64, Int32, Ptr{Float64}),

@@ -102,6 +102,7 @@ SIGNATURES = [
     ("pts_device_bytes", _i, [_vp, ctypes.POINTER(_i64)]),
     ("pts_placement_times", _i, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]),
     ("pts_set_knob", _i, [_vp, ctypes.c_char_p, ctypes.c_int64]),
+    ("pts_norms_dense_noalloc", _i, [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("pts_init_inclusion", _i, [_vp, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i32]),
     ("pts_init_grf_viscosity", _i, [_vp, _i32, _dp, _d, _d, _d, _d, _d, _d, _d, _d]),
     ("pts_smooth", _i, [_vp, _i, _i, _d]),
