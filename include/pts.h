@@ -114,7 +114,7 @@ int pts_placement_times(const pts_ctx* ctx, float* ms, int32_t n);
 /* No counterpart in the reference.  Sets one launch-configuration knob of
  * the single-pass kernels by the name of the environment variable that sets
  * it at pts_create (PTS_PLANES_PER_BLOCK, PTS_3D_VARIANT, PTS_3D_MAP_FLAT,
- * PTS_ROWS_PER_BLOCK, PTS_2D_VARIANT, PTS_2D_PERSIST), so
+ * PTS_3D_W64, PTS_ROWS_PER_BLOCK, PTS_2D_VARIANT, PTS_2D_PERSIST), so
  * that configurations can be compared on one set of device blocks.  Results
  * are bit-identical for every value; PTS_ERR_ARG for an unknown name. */
 int pts_set_knob(pts_ctx* ctx, const char* name, int64_t value);

@@ -52,6 +52,7 @@ struct pts_ctx {
     int variant_3d = 1;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
     bool group_3d = true;       // 3D: state fields, partners and τ_o in grouped allocations (ensure_group)
     bool map_flat_3d = false;   // k3_iter2: tiles in the flat (chunk, tile) order
+    bool w64_3d = false;        // k3_iter2: 64-bit plane bases also where 32-bit offsets fit
     // placement selection (place_fields): tries (3D, 2D), the least bytes of
     // field blocks for which it runs, a forced outcome for tests (-1: by time)
     int place_tries_3d = 5;
@@ -359,6 +360,7 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_3D_VARIANT")) c->variant_3d = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_GROUPS")) c->group_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_MAP_FLAT")) c->map_flat_3d = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_3D_W64")) c->w64_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_PLACE_TRIES")) c->place_tries_3d = std::min(31, std::max(0, std::atoi(e)));
     if (const char* e = std::getenv("PTS_2D_PLACE_TRIES")) c->place_tries_2d = std::min(31, std::max(0, std::atoi(e)));
     if (const char* e = std::getenv("PTS_2D_GROUPS")) c->group_2d = std::atoi(e) != 0;
@@ -871,7 +873,7 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s0, int64_t n, bool stor
             else
                 HIPCHK(c, launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
                                        c->tile_rows_3d, dp, c->variant_3d, c->slot_bytes, use_mx(c),
-                                       c->map_flat_3d));
+                                       c->map_flat_3d, c->w64_3d));
             swap_state(c);
         } else {
             if (store_last_dv && q == n - 1 && !mul) {
@@ -1113,6 +1115,7 @@ int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
     if (k == "PTS_PLANES_PER_BLOCK") c->planes_per_block = v;
     else if (k == "PTS_3D_VARIANT") c->variant_3d = v;
     else if (k == "PTS_3D_MAP_FLAT") c->map_flat_3d = v != 0;
+    else if (k == "PTS_3D_W64") c->w64_3d = v != 0;
     else if (k == "PTS_ROWS_PER_BLOCK") c->rows_per_block = v;
     else if (k == "PTS_2D_VARIANT") c->variant_2d = v;
     else if (k == "PTS_2D_PERSIST") c->persist_2d = v != 0;

@@ -203,10 +203,12 @@ hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 // 3D allocations (slot_bytes apart, with guard zones: see ensure_group and
 // field_guard_elems in pts_api.hip); otherwise k3_iter runs.  mx: form Musτ
 // from μ (k3_iter2 without prefetch); map_flat: k3_iter2 takes the tiles in
-// the flat (chunk, tile) order instead of each XCD sweeping its own tiles.
+// the flat (chunk, tile) order instead of each XCD sweeping its own tiles;
+// w64: k3_iter2 forms addresses from 64-bit plane bases even where 32-bit
+// offsets suffice (it does so by itself where they do not, e.g. at 1023^3).
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
                         bool mul, bool bf, int planes_per_block, int tile_rows, bool dp, int variant,
-                        int64_t slot_bytes, bool mx, bool map_flat);
+                        int64_t slot_bytes, bool mx, bool map_flat, bool w64);
 
 // Mixed-precision defect correction (pts_dc.hip)
 hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim);

@@ -401,7 +401,7 @@ struct Fields3g {
     const double* mu;
     const double* rh;                 // dτ_Rho (without dp)
     const double* rg;                 // ρg (body force)
-    uint32_t off_in, off_out, off_old;
+    uint64_t off_in, off_out, off_old;
     int sx, sxy, nx, ny, nz;
 };
 
@@ -464,7 +464,7 @@ __device__ __forceinline__ double tau_upd2(double t, double to, double g, double
     }
 }
 
-template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2, bool MX>
+template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2, bool MX, bool W64>
 __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const StepScalars cs, int Lz,
                                                     int ntx, int nty, int nchunk) {
     // new Pt1, τyy1 (read by row +1) and τxy1, τyz1 (read by row -1), by plane
@@ -525,28 +525,57 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
     const double* __restrict__ Vxb = f.base[PTS_VX] - bias;
     const double* __restrict__ Vyb = f.base[PTS_VY] - bias;
     const double* __restrict__ Vzb = f.base[PTS_VZ] - bias;
-    const double* __restrict__ Gp = f.g - bias;
-    const double* __restrict__ Mup = f.mu - bias;
-    const double* __restrict__ Rhp = f.rh - bias;
-    const double* __restrict__ rgp = f.rg - bias;
-    const uint32_t oin = f.off_in, oout = f.off_out, oold = f.off_old;
-
-    double* __restrict__ Ptm = const_cast<double*>(Ptb);
-    double* __restrict__ txxm = const_cast<double*>(txxb);
-    double* __restrict__ tyym = const_cast<double*>(tyyb);
-    double* __restrict__ tzzm = const_cast<double*>(tzzb);
-    double* __restrict__ txym = const_cast<double*>(txyb);
-    double* __restrict__ txzm = const_cast<double*>(txzb);
-    double* __restrict__ tyzm = const_cast<double*>(tyzb);
-    double* __restrict__ Vxm = const_cast<double*>(Vxb);
-    double* __restrict__ Vym = const_cast<double*>(Vyb);
-    double* __restrict__ Vzm = const_cast<double*>(Vzb);
+    const int k0 = max(kb - 1, 0), k1 = min(ke, nz - 1);
+    // W64: the slots of a field lie more than 4 GiB apart, so each slot gets
+    // its own pointer, moved to the block's first plane k0, and the per-lane
+    // offset counts planes from k0 (a march of at most a few hundred planes
+    // stays within 32 bits).  Otherwise one pointer per field and the slots
+    // as 32-bit offsets (the same pointer value for every slot).
+    const int64_t kshift = W64 ? (int64_t)k0 * sxy : 0;
+    auto slot = [&](const double* b, uint64_t off) __attribute__((always_inline)) {
+        return W64 ? b + (int64_t)(off >> 3) + kshift : b;
+    };
+    const double* __restrict__ Gp = f.g - bias + kshift;
+    const double* __restrict__ Mup = f.mu - bias + kshift;
+    const double* __restrict__ Rhp = f.rh - bias + kshift;
+    const double* __restrict__ rgp = f.rg - bias + kshift;
+    const uint32_t oin = W64 ? 0u : (uint32_t)f.off_in, oout = W64 ? 0u : (uint32_t)f.off_out;
+    const uint32_t oold = W64 ? 0u : (uint32_t)f.off_old;
+    // inputs (i), τ_o (l) and outputs (o) of the state fields
+    const double* __restrict__ Pt_i = slot(Ptb, f.off_in);
+    const double* __restrict__ Vx_i = slot(Vxb, f.off_in);
+    const double* __restrict__ Vy_i = slot(Vyb, f.off_in);
+    const double* __restrict__ Vz_i = slot(Vzb, f.off_in);
+    const double* __restrict__ txx_i = slot(txxb, f.off_in);
+    const double* __restrict__ tyy_i = slot(tyyb, f.off_in);
+    const double* __restrict__ tzz_i = slot(tzzb, f.off_in);
+    const double* __restrict__ txy_i = slot(txyb, f.off_in);
+    const double* __restrict__ txz_i = slot(txzb, f.off_in);
+    const double* __restrict__ tyz_i = slot(tyzb, f.off_in);
+    const double* __restrict__ txx_l = slot(txxb, f.off_old);
+    const double* __restrict__ tyy_l = slot(tyyb, f.off_old);
+    const double* __restrict__ tzz_l = slot(tzzb, f.off_old);
+    const double* __restrict__ txy_l = slot(txyb, f.off_old);
+    const double* __restrict__ txz_l = slot(txzb, f.off_old);
+    const double* __restrict__ tyz_l = slot(tyzb, f.off_old);
+    double* __restrict__ Ptm = const_cast<double*>(slot(Ptb, f.off_out));
+    double* __restrict__ txxm = const_cast<double*>(slot(txxb, f.off_out));
+    double* __restrict__ tyym = const_cast<double*>(slot(tyyb, f.off_out));
+    double* __restrict__ tzzm = const_cast<double*>(slot(tzzb, f.off_out));
+    double* __restrict__ txym = const_cast<double*>(slot(txyb, f.off_out));
+    double* __restrict__ txzm = const_cast<double*>(slot(txzb, f.off_out));
+    double* __restrict__ tyzm = const_cast<double*>(slot(tyzb, f.off_out));
+    double* __restrict__ Vxm = const_cast<double*>(slot(Vxb, f.off_out));
+    double* __restrict__ Vym = const_cast<double*>(slot(Vyb, f.off_out));
+    double* __restrict__ Vzm = const_cast<double*>(slot(Vzb, f.off_out));
 
     const double dx = cs.dx, dy = cs.dy, dz = cs.dz, _dx = cs._dx, _dy = cs._dy, _dz = cs._dz;
     const double r = cs.r;
-    const int k0 = max(kb - 1, 0), k1 = min(ke, nz - 1);
     // byte offset of (c, d, k): one 32-bit value per plane for every field
-    auto tko = [&](int k) __attribute__((always_inline)) { return tb + (uint32_t)k * (uint32_t)sxy * 8u; };
+    // (W64: planes counted from k0)
+    auto tko = [&](int k) __attribute__((always_inline)) {
+        return tb + (uint32_t)(W64 ? k - k0 : k) * (uint32_t)sxy * 8u;
+    };
 
     const uint32_t sxy8 = (uint32_t)sxy * 8u;
 
@@ -581,28 +610,28 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
         } else {
             I.mu = ldb(Mup, t);
         }
-        I.vx = ldb(Vxb, ti);
-        I.vy = ldb(Vyb, ti);
-        I.vzn = ldb(Vzb, ti + sxy8);
-        if constexpr (R != 2) I.pt = ldb(Ptb, ti);
+        I.vx = ldb(Vx_i, ti);
+        I.vy = ldb(Vy_i, ti);
+        I.vzn = ldb(Vz_i, ti + sxy8);
+        if constexpr (R != 2) I.pt = ldb(Pt_i, ti);
         if constexpr (BF && R == 1) I.rg = ldb(rgp, t);
         if constexpr (R == 1) {
-            I.tx = ldb(txxb, ti);
-            if constexpr (VE) I.txo = ldb(txxb, to);
-            I.tz = ldb(tzzb, ti);
-            if constexpr (VE) I.tzo = ldb(tzzb, to);
-            I.txz = ldb(txzb, ti);
-            if constexpr (VE) I.txzo = ldb(txzb, to);
+            I.tx = ldb(txx_i, ti);
+            if constexpr (VE) I.txo = ldb(txx_l, to);
+            I.tz = ldb(tzz_i, ti);
+            if constexpr (VE) I.tzo = ldb(tzz_l, to);
+            I.txz = ldb(txz_i, ti);
+            if constexpr (VE) I.txzo = ldb(txz_l, to);
         }
         if constexpr (R != 2) {
-            I.ty = ldb(tyyb, ti);
-            if constexpr (VE) I.tyo = ldb(tyyb, to);
+            I.ty = ldb(tyy_i, ti);
+            if constexpr (VE) I.tyo = ldb(tyy_l, to);
         }
         if constexpr (R != 0) {
-            I.txy = ldb(txyb, ti);
-            if constexpr (VE) I.txyo = ldb(txyb, to);
-            I.tyz = ldb(tyzb, ti);
-            if constexpr (VE) I.tyzo = ldb(tyzb, to);
+            I.txy = ldb(txy_i, ti);
+            if constexpr (VE) I.txyo = ldb(txy_l, to);
+            I.tyz = ldb(tyz_i, ti);
+            if constexpr (VE) I.tyzo = ldb(tyz_l, to);
         }
     };
     // Gdτ, dτ_Rho of plane k (DP: from Musτ) and the exchange with the
@@ -835,7 +864,7 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
         Nbr3 U{};
         Carry3 P{};
         load(role, k0, A, true);
-        const double vz0 = ldb(Vzb, tko(k0) + oin);  // Vz at z-face k0
+        const double vz0 = ldb(Vz_i, tko(k0) + oin);  // Vz at z-face k0
         P.vzC = vz0;
         exchange(role, k0, A, U, P, true, vz0, P.vzUC);
         for (int k = k0;;) {
@@ -852,42 +881,57 @@ __global__ __launch_bounds__(64 * BY) void k3_iter2(const Fields3g f, const Step
     else run(std::integral_constant<int, 1>{});
 }
 
-template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2, bool MX>
+template <bool VE, bool MUL, bool BF, int BY, bool DP, bool PF, bool G2, bool MX, bool W64>
 int concurrency3b() {
     static int cached = 0;
     if (!cached) {
         int dev = 0, cus = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k3_iter2<VE, MUL, BF, BY, DP, PF, G2, MX>, 64 * BY, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k3_iter2<VE, MUL, BF, BY, DP, PF, G2, MX, W64>, 64 * BY, 0);
         cached = std::max(1, cus) * std::max(1, per);
     }
     return cached;
 }
 
-template <bool VE, bool MUL, bool BF, bool DP, bool PF, bool G2, bool MX = false>
+template <bool VE, bool MUL, bool BF, bool DP, bool PF, bool G2, bool MX = false, bool W64 = false>
 void go3b(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bool map_flat) {
     constexpr int BY = 12;
+    if constexpr (W64) {  // the per-lane offset spans the march: (L + 4) planes within 2^31 bytes
+        const int64_t cap = std::max<int64_t>(1, ((int64_t)1 << 31) / ((int64_t)in.sxy * 8) - 4);
+        Lz = Lz > 0 ? (int)std::min<int64_t>(Lz, cap) : -(int)std::min<int64_t>(-(int64_t)Lz, cap);
+    }
     const int ntx = (in.nx + 1 + 61) / 62;
     const int nty = (in.ny + 1 + (BY - 2) - 1) / (BY - 2);
-    const int64_t conc = concurrency3b<VE, MUL, BF, BY, DP, PF, G2, MX>();
+    const int64_t conc = concurrency3b<VE, MUL, BF, BY, DP, PF, G2, MX, W64>();
     if (map_flat) {
         const int L = planes_per_block(Lz, in.nz, (int64_t)ntx * nty, conc);
         const int nch = (in.nz + L - 1) / L;
-        k3_iter2<VE, MUL, BF, BY, DP, PF, G2, MX><<<(unsigned)(ntx * nty * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, 0);
+        k3_iter2<VE, MUL, BF, BY, DP, PF, G2, MX, W64><<<(unsigned)(ntx * nty * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, 0);
         return;
     }
     // per XCD: ceil(tiles / 8) tiles per chunk on an eighth of the resident blocks
     const int64_t per = ((int64_t)ntx * nty + 7) / 8;
     const int L = planes_per_block(Lz, in.nz, per, std::max<int64_t>(1, conc / 8));
     const int nch = (in.nz + L - 1) / L;
-    k3_iter2<VE, MUL, BF, BY, DP, PF, G2, MX><<<(unsigned)(8 * per * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, nch);
+    k3_iter2<VE, MUL, BF, BY, DP, PF, G2, MX, W64><<<(unsigned)(8 * per * nch), dim3(64, BY), 0, s>>>(in, c, L, ntx, nty, nch);
 }
 
 template <bool VE, bool MUL, bool BF, bool DP>
 void go3b_pf(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bool pf, bool g2, bool mx,
-             bool map_flat) {
+             bool map_flat, bool w64) {
     if constexpr (DP) {
+        // slots more than 4 GiB apart: per-slot pointers (without prefetch)
+        if (w64) {
+            if (mx) {
+                if (g2) go3b<VE, MUL, BF, DP, false, true, true, true>(s, in, c, Lz, map_flat);
+                else go3b<VE, MUL, BF, DP, false, false, true, true>(s, in, c, Lz, map_flat);
+            } else {
+                if (g2) go3b<VE, MUL, BF, DP, false, true, false, true>(s, in, c, Lz, map_flat);
+                else go3b<VE, MUL, BF, DP, false, false, false, true>(s, in, c, Lz, map_flat);
+            }
+            return;
+        }
         if (mx && pf) {
             if (g2) go3b<VE, MUL, BF, DP, true, true, true>(s, in, c, Lz, map_flat);
             else go3b<VE, MUL, BF, DP, true, false, true>(s, in, c, Lz, map_flat);
@@ -910,73 +954,80 @@ void go3b_pf(hipStream_t s, const Fields3g& in, const StepScalars& c, int Lz, bo
 
 // The grouped view of a ping-pong pair, if its buffers are laid out as slots
 // of one allocation `slot` bytes apart (in and out in slots 0/1 in either
-// order, τ_o in slot 2) and every offset fits in 32 bits.
-bool group_fields(const Fields& in, const Fields& out, bool ve, bool dp, bool bf, int64_t slot, Fields3g& g) {
-    if (slot <= 0) return false;
+// order, τ_o in slot 2).  0: not grouped; 1: every offset fits in 32 bits;
+// 2: per-slot pointers are needed (or asked for by w64), which only asks
+// that a few planes fit in 31 bits.
+int group_fields(const Fields& in, const Fields& out, bool ve, bool dp, bool bf, int64_t slot, bool w64,
+                 Fields3g& g) {
+    if (slot <= 0) return 0;
     g = Fields3g{};
     int par = -1;
     for (int f = PTS_PT; f <= PTS_TYZ; ++f) {
         const char* a = (const char*)in.p[f];
         const char* b = (const char*)out.p[f];
-        if (!a || !b) return false;
+        if (!a || !b) return 0;
         int p;
         if (b - a == slot) p = 0;
         else if (a - b == slot) p = 1;
-        else return false;
-        if (par >= 0 && p != par) return false;
+        else return 0;
+        if (par >= 0 && p != par) return 0;
         par = p;
         const char* base = p ? b : a;
         g.base[f] = (const double*)base;
-        if (ve && f >= PTS_TXX && (const char*)in.p[f + (PTS_TXX_O - PTS_TXX)] != base + 2 * slot) return false;
+        if (ve && f >= PTS_TXX && (const char*)in.p[f + (PTS_TXX_O - PTS_TXX)] != base + 2 * slot) return 0;
     }
     g.g = dp ? in.p[PTS_MUSTAU] : in.p[PTS_GDTAU];
     g.mu = in.p[PTS_MUS];
     g.rh = in.p[PTS_DTAU_RHO];
     g.rg = in.p[PTS_RHOG];
-    if (!g.g || !g.mu || (!dp && !g.rh) || (bf && !g.rg)) return false;
+    if (!g.g || !g.mu || (!dp && !g.rh) || (bf && !g.rg)) return 0;
     // largest byte offset: the last slot plus the box, one plane of Vz beyond
     // and the guard zone (k3_iter2 stays inside the guard zones)
     const int64_t box = ((int64_t)in.sxy * (in.nz + 1) + 32 * (int64_t)in.sx + 256) * (int64_t)sizeof(double);
-    if ((ve ? 2 : 1) * slot + box >= ((int64_t)1 << 32)) return false;
-    g.off_in = (uint32_t)(par ? slot : 0);
-    g.off_out = (uint32_t)(par ? 0 : slot);
-    g.off_old = (uint32_t)(2 * slot);
+    const bool fits32 = (ve ? 2 : 1) * slot + box < ((int64_t)1 << 32);
+    if (!fits32 && 5 * (int64_t)in.sxy * (int64_t)sizeof(double) >= ((int64_t)1 << 31)) return 0;
+    g.off_in = (uint64_t)(par ? slot : 0);
+    g.off_out = (uint64_t)(par ? 0 : slot);
+    g.off_old = (uint64_t)(2 * slot);
     g.sx = in.sx;
     g.sxy = in.sxy;
     g.nx = in.nx;
     g.ny = in.ny;
     g.nz = in.nz;
-    return true;
+    return (fits32 && !w64) ? 1 : 2;
 }
 
 }  // namespace
 
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve, bool mul,
                         bool bf, int Lz, int by, bool dp, int variant, int64_t slot_bytes, bool mx,
-                        bool map_flat) {
+                        bool map_flat, bool w64) {
     if (Lz == 0) Lz = 1;
     Fields3g g;
-    if ((variant == 1 || variant == 2) && group_fields(in, out, ve, dp, bf, slot_bytes, g)) {
+    const int gm = (variant == 1 || variant == 2) ? group_fields(in, out, ve, dp, bf, slot_bytes, w64, g) : 0;
+    // per-slot pointers are instantiated for the derived-parameter kernels
+    if (gm == 1 || (gm == 2 && dp)) {
         const bool pf = variant == 2;
+        const bool wide = gm == 2;
         int e2 = 0;
         const bool g2 = ve && std::frexp(c.Gdt, &e2) == 0.5;  // G·dt a power of two
         StepScalars c2 = c;
         c2.iGdt = g2 ? 1.0 / c.Gdt : 0.0;
         if (ve) {
             if (mul) {
-                if (dp) { if (bf) go3b_pf<true, true, true, true>(s, g, c2, Lz, pf, g2, mx, map_flat); else go3b_pf<true, true, false, true>(s, g, c2, Lz, pf, g2, mx, map_flat); }
-                else { if (bf) go3b_pf<true, true, true, false>(s, g, c2, Lz, pf, g2, mx, map_flat); else go3b_pf<true, true, false, false>(s, g, c2, Lz, pf, g2, mx, map_flat); }
+                if (dp) { if (bf) go3b_pf<true, true, true, true>(s, g, c2, Lz, pf, g2, mx, map_flat, wide); else go3b_pf<true, true, false, true>(s, g, c2, Lz, pf, g2, mx, map_flat, wide); }
+                else { if (bf) go3b_pf<true, true, true, false>(s, g, c2, Lz, pf, g2, mx, map_flat, wide); else go3b_pf<true, true, false, false>(s, g, c2, Lz, pf, g2, mx, map_flat, wide); }
             } else {
-                if (dp) { if (bf) go3b_pf<true, false, true, true>(s, g, c2, Lz, pf, g2, mx, map_flat); else go3b_pf<true, false, false, true>(s, g, c2, Lz, pf, g2, mx, map_flat); }
-                else { if (bf) go3b_pf<true, false, true, false>(s, g, c2, Lz, pf, g2, mx, map_flat); else go3b_pf<true, false, false, false>(s, g, c2, Lz, pf, g2, mx, map_flat); }
+                if (dp) { if (bf) go3b_pf<true, false, true, true>(s, g, c2, Lz, pf, g2, mx, map_flat, wide); else go3b_pf<true, false, false, true>(s, g, c2, Lz, pf, g2, mx, map_flat, wide); }
+                else { if (bf) go3b_pf<true, false, true, false>(s, g, c2, Lz, pf, g2, mx, map_flat, wide); else go3b_pf<true, false, false, false>(s, g, c2, Lz, pf, g2, mx, map_flat, wide); }
             }
         } else {
             if (mul) {
-                if (dp) { if (bf) go3b_pf<false, true, true, true>(s, g, c2, Lz, pf, false, mx, map_flat); else go3b_pf<false, true, false, true>(s, g, c2, Lz, pf, false, mx, map_flat); }
-                else { if (bf) go3b_pf<false, true, true, false>(s, g, c2, Lz, pf, false, mx, map_flat); else go3b_pf<false, true, false, false>(s, g, c2, Lz, pf, false, mx, map_flat); }
+                if (dp) { if (bf) go3b_pf<false, true, true, true>(s, g, c2, Lz, pf, false, mx, map_flat, wide); else go3b_pf<false, true, false, true>(s, g, c2, Lz, pf, false, mx, map_flat, wide); }
+                else { if (bf) go3b_pf<false, true, true, false>(s, g, c2, Lz, pf, false, mx, map_flat, wide); else go3b_pf<false, true, false, false>(s, g, c2, Lz, pf, false, mx, map_flat, wide); }
             } else {
-                if (dp) { if (bf) go3b_pf<false, false, true, true>(s, g, c2, Lz, pf, false, mx, map_flat); else go3b_pf<false, false, false, true>(s, g, c2, Lz, pf, false, mx, map_flat); }
-                else { if (bf) go3b_pf<false, false, true, false>(s, g, c2, Lz, pf, false, mx, map_flat); else go3b_pf<false, false, false, false>(s, g, c2, Lz, pf, false, mx, map_flat); }
+                if (dp) { if (bf) go3b_pf<false, false, true, true>(s, g, c2, Lz, pf, false, mx, map_flat, wide); else go3b_pf<false, false, false, true>(s, g, c2, Lz, pf, false, mx, map_flat, wide); }
+                else { if (bf) go3b_pf<false, false, true, false>(s, g, c2, Lz, pf, false, mx, map_flat, wide); else go3b_pf<false, false, false, false>(s, g, c2, Lz, pf, false, mx, map_flat, wide); }
             }
         }
         return hipGetLastError();

@@ -273,6 +273,15 @@ VARIANTS = [
     ("Stokes3D_ve_multixpu", dict(n=13, rhog_in=-1.0), {"PTS_DERIVED_MUSTAU": "0"}),
     ("Stokes3D_ve_perf", dict(nx=3, ny=5, nz=4), {}),
     ("Stokes3D_ve_perf", dict(nx=140, ny=30, nz=45), {"PTS_3D_MAP_FLAT": "1"}),
+    # per-slot 64-bit pointers (forced here; chosen by themselves where the
+    # slots lie more than 4 GiB apart, e.g. at 1023^3)
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=30), {"PTS_3D_W64": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=125, ny=31, nz=9, dt=0.3), {"PTS_3D_W64": "1"}),
+    ("Stokes3D_ve_multixpu", dict(n=19, rhog_in=-1.0), {"PTS_3D_W64": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=63, ny=11, nz=30, ve=False, rhog_in=-1.0), {"PTS_3D_W64": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_3D_W64": "1", "PTS_DERIVED_MUSTAU": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=80), {"PTS_3D_W64": "1", "PTS_PLANES_PER_BLOCK": "64"}),
+    ("Stokes3D_ve_perf", dict(nx=5, ny=4, nz=3), {"PTS_3D_W64": "1"}),
     ("Stokes3D_ve_perf", dict(nx=125, ny=22, nz=30, ve=False), {"PTS_PLANES_PER_BLOCK": "5"}),
     # placement selection moved into every size: blocks moved on every try,
     # never, or when faster
