@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
 """Benchmark of the PT iteration on one MI355X.
 
-A "step" is one PT iteration (the single-pass kernel in 2D: Pt/τ update,
-V update and free-slip copies for the whole grid).  The headline workload is
-2D Maxwell visco-elastic Stokes at nx = ny = 8191 in FP64 (dt = η/G/10),
-reported as the reference's effective throughput T_eff = A_eff / t_it with
-A_eff = 18·nx·ny·8 B (scripts/Stokes2D_ve_perf.jl:173: 6 fields read and
-written + 6 read-only).  The 3D VE 511³ block (A_eff = 29·n³·8 B,
-Stokes3D_ve_perf.jl:229) is reported alongside.
+A "step" is one PT iteration (the single-pass kernel: Pt/τ update, V update
+and free-slip copies for the whole grid).  The headline workload is the 3D
+Maxwell visco-elastic block at nx = ny = nz = 511 in FP64 (G = 1, dt = 1,
+x-z pure shear, 1e-3 sphere), reported as the reference's effective
+throughput T_eff = A_eff / t_it with A_eff = 29·n³·8 B
+(scripts/stokes_3D/Stokes3D_ve_perf.jl:229: 10 fields read and written + 9
+read-only).  2D Maxwell VE at nx = ny = 8191 (dt = η/G/10, A_eff = 18·n²·8 B,
+Stokes2D_ve_perf.jl:173) is reported alongside.
 
 Timing: HIP events recorded on the library's own stream around exactly
 `--steps` iterations, after `--warmup` untimed ones, with the stream drained
@@ -28,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "pseudotransientstokes.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
 A100_2D_VE_8191_GBS = 932.8  # output/out_Stokes2D_ve_perf_ampere.txt:45
+A100_3D_VE_511_GBS = 732.2   # output/out_Stokes3D_ve_perf_ampere.txt:15
 METRIC = "T_eff GB/s (% of HBM peak) + time-to-‖res‖∞<1e-8, FP32-refined vs FP64"
 
 
@@ -175,7 +177,8 @@ def bench_tts(n, nout=500, rtol_inner=1e-4, use_graph=True):
         rel[f] = float(np.max(np.abs(a - v)) / np.max(np.abs(v)))
     m.close()
     return dict(workload=f"3D viscous Stokes, log10 mu GRF (seed 1234, corr. 0.5, sigma 1, clip 2) + 1e-3 sphere "
-                         f"with rhog_z=-1, nx=ny=nz={n}, stop ||res||_inf < 1e-8 (scaled max-norm)",
+                         f"with rhog_z=-1, mu smoothed by 20 passes (10 leave the PT iteration unstable on this "
+                         f"contrast), nx=ny=nz={n}, stop ||res||_inf < 1e-8 (scaled max-norm)",
                 fp64_s=round(t64, 3), fp64_iters=r64["iters"], fp64_err=r64["err"],
                 dc_s=round(tdc, 3), dc_outer=rdc["outer"], dc_fp32_sweeps=rdc["inner_iters"], dc_err=rdc["err"],
                 dc_converged=rdc["converged"], time_ratio_dc_over_fp64=round(tdc / t64, 3),
@@ -236,13 +239,16 @@ def bench_c5_grad(n, true_seed=4321, seed=1234, chunk=20000):
     """Time to ∂J/∂ln μ of the 2D viscous inversion: ln μ a seeded Gaussian
     random field, J = ½Σ|Vx − Vx_obs|² at 256 seeded points on the top
     boundary, observations from a hidden ln μ (one forward solve).  Then a
-    forward and an adjoint PT solve, each to the scaled max-norm 1e-10, and
-    the gradient kernel; each phase timed on its own.  The forward solves run
+    forward PT solve to the reference's err (scaled RMS) 1e-10, the adjoint
+    PT solve to a relative change of 1e-10, and the gradient kernel; each
+    phase timed on its own.  The forward solves run
     in chunks of `chunk` iterations (a multiple of the check interval, so the
     checks and the result are those of one loop) to report progress."""
     import numpy as np
     from ptstokes.adjoint import Inversion
-    inv = Inversion(n=n, npts=256, eps=1e-10, adj_eps=1e-10)
+    # the reference's err (scaled RMS): the scaled max-norm has a roundoff floor
+    # above 1e-10 at 2047^2 (≈3e-11 at 511^2, growing with n)
+    inv = Inversion(n=n, npts=256, eps=1e-10, adj_eps=1e-10, norm_kind=0)
     truth, m0 = inv.grf_logmu(true_seed), inv.grf_logmu(seed)
 
     def forward(label):
@@ -273,7 +279,8 @@ def bench_c5_grad(n, true_seed=4321, seed=1234, chunk=20000):
     adj = inv.last["adjoint"]
     print(f"# C5 {n}^2: adjoint + gradient {t_adj:.2f} s, {adj['iters']} iterations", file=sys.stderr, flush=True)
     out = dict(workload=f"2D viscous inversion, nx=ny={n}, ln mu GRF (seed {seed}; observations from seed "
-                        f"{true_seed}), 256 top-boundary Vx points, forward and adjoint PT to ||res||_inf < 1e-10",
+                        f"{true_seed}), 256 top-boundary Vx points, forward PT to err (scaled RMS) < 1e-10, adjoint PT "
+                        f"to a relative change < 1e-10",
                observe_s=round(t_obs, 3), observe_iters=obs["iters"], forward_s=round(t_fwd, 3),
                forward_iters=fwd["iters"], forward_err=fwd["err"], adjoint_gradient_s=round(t_adj, 3),
                adjoint_iters=adj["iters"], adjoint_err=adj["err"], J=J, grad_max_abs=float(np.max(np.abs(grad))),
@@ -325,7 +332,8 @@ def bench_c4_kernels(n, ev):
     bytes32 = m.ctx.device_bytes()
     m.close()
     print(f"# 3D GRF {n}^3 FP32 correction sweep {ms32:.3f} ms", file=sys.stderr, flush=True)
-    return dict(workload=f"3D viscous Stokes, log10 mu GRF + 1e-3 sphere with rhog_z=-1, nx=ny=nz={n}",
+    return dict(workload=f"3D viscous Stokes, log10 mu GRF + 1e-3 sphere with rhog_z=-1, 20 smoothing passes, "
+                         f"nx=ny=nz={n}",
                 fp64_ms_per_it=round(ms64, 4), fp64_A_eff_bytes=a_eff, fp64_T_eff_GBs=round(a_eff / ms64 / 1e6, 1),
                 fp32_ms_per_sweep=round(ms32, 4), fp32_A_eff_bytes=a_eff / 2,
                 fp32_T_eff_GBs=round(a_eff / 2 / ms32 / 1e6, 1), sweep_over_iteration=round(ms32 / ms64, 3),
@@ -397,64 +405,118 @@ def bench_c5(n=2047, steps=200):
                                        "the dJ/dVx source, and writes 6")
 
 
-def cpu_baseline(n, iters, threads):
+def host_cpu():
+    """Threads the CPU baseline uses (the CPUs this process may run on, or
+    OMP_NUM_THREADS where that is set lower: the GPU box sets it to its CPU
+    share), the physical cores among them and the CPU model."""
+    cpus = sorted(os.sched_getaffinity(0))
+    threads = len(cpus)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    model, cores = "unknown", set()
+    try:
+        blocks = open("/proc/cpuinfo").read().strip().split("\n\n")
+        for b in blocks:
+            kv = dict(line.split(":", 1) for line in b.splitlines() if ":" in line)
+            kv = {k.strip(): v.strip() for k, v in kv.items()}
+            if int(kv.get("processor", -1)) in cpus:
+                model = kv.get("model name", model)
+                cores.add((kv.get("physical id", "0"), kv.get("core id", kv.get("processor"))))
+    except (OSError, ValueError):
+        pass
+    return dict(threads=threads, affinity_cpus=len(cpus), physical_cores=len(cores) or None, model=model,
+                omp_num_threads=omp or None)
+
+
+def _timed_best(o, warm, timed, repeats):
+    """The reference's protocol (Stokes2D_ve_perf.jl:148,172-176; runme_perf.sh:48):
+    the clock starts after the first `warm` iterations; the best of `repeats`
+    runs of `timed` iterations."""
+    o.iterate(warm)
+    best = math.inf
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        o.iterate(timed)
+        best = min(best, (time.perf_counter() - t0) / timed)
+    return best
+
+
+def cpu_baseline_2d(n, threads, warm=10, timed=91, repeats=5):
     """The oracle (the reference's CPU kernels restated in C, OpenMP over the
-    stencil loops) timed on the host cores at the same workload."""
+    stencil loops) on 2D VE n², timed as the reference's perf script times
+    itself: iterations 11..101 of one physical step, best of 5 repeats."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     o = Oracle("Stokes2D_ve_perf", n=n, dt=0.1, threads=threads).setup()
     o.assign_tau()
-    o.iterate(1)  # first touch
-    t0 = time.perf_counter()
-    o.iterate(iters)
-    t = (time.perf_counter() - t0) / iters
+    t = _timed_best(o, warm, timed, repeats)
     a_eff = 18.0 * n * n * 8
-    return dict(value=a_eff / t / 1e9, unit="GB/s", cores=threads, kind="port",
-                sample=f"{iters} PT iterations of 2D VE {n}x{n} FP64 (perf-script kernels, "
-                       f"{threads} OpenMP threads) after setup; ms/it {t * 1e3:.1f}")
+    return dict(value=a_eff / t / 1e9, unit="GB/s", cores=threads, kind="port", ms_per_it=t * 1e3,
+                sample=f"2D VE {n}x{n} FP64 (perf-script kernels, {threads} OpenMP threads): iterations "
+                       f"{warm + 1}..{warm + timed} of one step, best of {repeats} repeats; ms/it {t * 1e3:.1f}")
 
 
-def cpu_baseline_3d(n, iters, threads):
-    """Per-iteration throughput of the oracle on the 3D VE block."""
+def cpu_baseline_3d(n, threads, warm=10, timed=10, repeats=3):
+    """The oracle on the 3D VE block: the clock starts at iteration 11 as in
+    the reference's perf script; a bounded sample (10 iterations, best of 3)
+    instead of its 190 (one iteration takes a quarter second on the host)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
     t0 = time.perf_counter()
     o = Oracle("Stokes3D_ve_perf", n=n, threads=threads, nsmooth=1).setup()
     t_setup = time.perf_counter() - t0
     o.assign_tau()
-    o.iterate(1)
-    t0 = time.perf_counter()
-    o.iterate(iters)
-    t = (time.perf_counter() - t0) / iters
+    t = _timed_best(o, warm, timed, repeats)
     a_eff = 29.0 * n ** 3 * 8
     return dict(value=a_eff / t / 1e9, unit="GB/s", cores=threads, kind="port", ms_per_it=t * 1e3,
-                sample=f"{iters} PT iterations of 3D VE {n}^3 FP64 (perf-script kernels, {threads} OpenMP threads) "
-                       f"after setup ({t_setup:.0f} s, 1 smoothing pass)")
+                sample=f"3D VE {n}^3 FP64 (perf-script kernels, {threads} OpenMP threads): iterations "
+                       f"{warm + 1}..{warm + timed} of one step, best of {repeats} repeats, after the set-up "
+                       f"({t_setup:.0f} s, 1 smoothing pass); ms/it {t * 1e3:.1f}")
 
 
-def load_traffic(key):
+# The kernel sources each HBM-traffic record was measured with: a record only
+# describes a build of the same sources.
+TRAFFIC_SOURCES = {"k3_iter2": ["pts_iter3d.hip", "pts_internal.hpp"], "k2_iter": ["pts_step2d.hip", "pts_internal.hpp"]}
+
+
+def sources_sha256(kernel):
+    import hashlib
+    h = hashlib.sha256()
+    for f in TRAFFIC_SOURCES[kernel]:
+        with open(os.path.join(ROOT, "pseudotransientstokes.jl_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def load_traffic(kernel, n):
+    """Measured HBM bytes per launch (profiles/hbm_traffic.json) and a note;
+    None when the record is missing or was taken with other kernel sources."""
     p = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         with open(p) as f:
-            return json.load(f).get(key)
+            rec = json.load(f).get("kernels", {}).get(f"{kernel}_{n}")
     except (OSError, ValueError):
-        return None
+        rec = None
+    if not rec:
+        return None, f"no measured traffic for {kernel} at n={n}"
+    if rec.get("sources_sha256") != sources_sha256(kernel):
+        return None, f"traffic of {kernel} at n={n} was measured with other kernel sources (stale)"
+    return rec, rec.get("profile", "profiles/hbm_traffic.json")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n2d", type=int, default=8191)
     ap.add_argument("--n3d", type=int, default=511)
     ap.add_argument("--contexts2d", type=int, default=3, help="2D: contexts timed one after another (median reported)")
     ap.add_argument("--contexts3d", type=int, default=3, help="3D: fresh allocations timed (median reported)")
-    ap.add_argument("--no-3d", action="store_true")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2D VE line")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--n-tts", type=int, default=127, help="time-to-tolerance grid (0: skip)")
-    ap.add_argument("--no-cpu-3d", action="store_true", help="skip the oracle's time on the 3D block")
     ap.add_argument("--no-extra", action="store_true", help="skip the 127^2 and 2047^2 extra lines")
     ap.add_argument("--c3-solve-steps", type=int, default=0,
                     help="also run that many converged physical steps of the 3D VE block (minutes)")
@@ -462,9 +524,7 @@ def main():
                     help="also time the FP64 iteration and FP32 sweep of the 3D random-viscosity problem at this n "
                          "(1023: ~225 GB of device memory)")
     ap.add_argument("--c5-grad", type=int, default=0,
-                    help="also time the 2D inversion's forward solve, adjoint solve and gradient at this n "
-                         "(2047: each solve to 1e-10 takes more than 3 minutes; the forward solves print per chunk, "
-                         "the adjoint solve when it ends)")
+                    help="also time the 2D inversion's forward solve, adjoint solve and gradient at this n")
     ap.add_argument("--c2-solve-steps", type=int, default=0,
                     help="also run that many converged physical steps of 2D VE 8191^2 (minutes each)")
     ap.add_argument("--tts-no-graph", action="store_true",
@@ -476,22 +536,37 @@ def main():
         return 2
 
     ev = HipEvents()
-    r2 = bench_2d(args.n2d, args.steps, args.warmup, ev, args.contexts2d)
-    print(f"# 2D VE {args.n2d}^2 placements tried (ms): {r2['placement_ms_by_context']} "
-          f"({r2['placement_s_by_context']} s); contexts {r2['ms_per_it_by_context']} ms/it", file=sys.stderr)
-    print(f"# 2D VE {args.n2d}^2: {r2['ms_per_it']:.4f} ms/it, T_eff {r2['T_eff_GBs']:.1f} GB/s", file=sys.stderr,
+    host = host_cpu()
+    # headline: the 3D VE block
+    r3 = bench_3d(args.n3d, args.steps, args.warmup, ev, args.contexts3d)
+    print(f"# 3D VE {args.n3d}^3 placements tried (ms): {r3['placement_ms_by_context']}", file=sys.stderr)
+    print(f"# 3D VE {args.n3d}^3: {r3['ms_per_it']:.4f} ms/it, T_eff {r3['T_eff_GBs']:.1f} GB/s", file=sys.stderr,
           flush=True)
-    extra = {}
-    if not args.no_3d:
-        s3 = max(20, args.steps // 5)
-        r3 = bench_3d(args.n3d, s3, max(5, args.warmup // 2), ev, args.contexts3d)
-        r3["frac_of_peak"] = r3["T_eff_GBs"] / HBM_PEAK_GBS
-        r3["workload"] = f"3D Maxwell VE Stokes, nx=ny=nz={args.n3d}, FP64, x-z pure shear"
-        r3["hbm_traffic_bytes"] = load_traffic(f"k3_iter2_{args.n3d}")
-        extra["3d_ve"] = r3
-        print(f"# 3D VE {args.n3d}^3 placements tried (ms): {r3['placement_ms_by_context']}", file=sys.stderr)
-        print(f"# 3D VE {args.n3d}^3: {r3['ms_per_it']:.4f} ms/it, T_eff {r3['T_eff_GBs']:.1f} GB/s",
+    extra = {"host_cpu": host}
+    traffic3, traffic3_note = load_traffic("k3_iter2", args.n3d)
+    extra["3d_ve"] = dict(ms_per_it_by_context=r3["ms_per_it_by_context"],
+                          placement_ms_by_context=r3["placement_ms_by_context"],
+                          placement_s_by_context=r3["placement_s_by_context"], device_bytes=r3["device_bytes"],
+                          wall_ms_per_it=r3["wall_ms_per_it"], traffic_record=traffic3_note)
+    if traffic3:
+        extra["3d_ve"].update(hbm_read_bytes=traffic3.get("read_bytes"), hbm_write_bytes=traffic3.get("write_bytes"),
+                              achieved_hbm_GBs=round(traffic3["bytes"] / (r3["ms_per_it"] / 1e3) / 1e9, 1))
+    if not args.no_2d:
+        r2 = bench_2d(args.n2d, args.steps, args.warmup, ev, args.contexts2d)
+        print(f"# 2D VE {args.n2d}^2 placements tried (ms): {r2['placement_ms_by_context']} "
+              f"({r2['placement_s_by_context']} s); contexts {r2['ms_per_it_by_context']} ms/it", file=sys.stderr)
+        print(f"# 2D VE {args.n2d}^2: {r2['ms_per_it']:.4f} ms/it, T_eff {r2['T_eff_GBs']:.1f} GB/s",
               file=sys.stderr, flush=True)
+        r2["frac_of_peak"] = r2["T_eff_GBs"] / HBM_PEAK_GBS
+        r2["vs_a100"] = round(r2["T_eff_GBs"] / A100_2D_VE_8191_GBS, 3)
+        r2["workload"] = (f"2D Maxwell visco-elastic Stokes PT iteration, nx=ny={args.n2d}, FP64, G=eta=1, dt=0.1, "
+                          "pure shear, 1e-3 inclusion r=1 (single-pass kernel)")
+        t2, t2_note = load_traffic("k2_iter", args.n2d)
+        r2["traffic_record"] = t2_note
+        if t2:
+            r2["hbm_bytes"] = t2["bytes"]
+            r2["achieved_hbm_GBs"] = round(t2["bytes"] / (r2["ms_per_it"] / 1e3) / 1e9, 1)
+        extra["2d_ve"] = r2
     if args.n_tts > 0:
         tts = bench_tts(args.n_tts, use_graph=not args.tts_no_graph)
         extra["tts_fp32_refined_vs_fp64"] = tts
@@ -503,7 +578,7 @@ def main():
         print(f"# 2047^2 forward {c5['forward_ms_per_it']} ms/it, adjoint {c5['adjoint_ms_per_it']} ms/it",
               file=sys.stderr, flush=True)
         if not args.no_cpu:
-            c1 = bench_c1(threads=min(16, os.cpu_count() or 1))
+            c1 = bench_c1(threads=host["threads"])
             extra["c1_viscous_127_time_to_solution"] = c1
             print(f"# 127^2 viscous: GPU {c1['gpu_s']} s, CPU {c1['cpu_s']} s ({c1['gpu_iters']} iterations)",
                   file=sys.stderr, flush=True)
@@ -522,21 +597,17 @@ def main():
         print(f"# 3D VE {args.n3d}^3 time to solution: {c3s['iterations']} iterations in {c3s['wall_s']} s",
               file=sys.stderr, flush=True)
     cpu = None
-    if not args.no_cpu and not args.no_3d and not args.no_cpu_3d:
-        threads = min(16, os.cpu_count() or 1)
-        c3 = cpu_baseline_3d(args.n3d, 2, threads)
-        extra["3d_ve"]["cpu_baseline"] = c3
-        extra["3d_ve"]["gpu_over_cpu"] = round(extra["3d_ve"]["T_eff_GBs"] / c3["value"], 1)
-        print(f"# CPU 3D: {c3['value']:.2f} GB/s ({c3['sample']})", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(args.n2d, args.cpu_iters, threads)
-        print(f"# CPU baseline: {cpu['value']:.2f} GB/s ({cpu['sample']})", file=sys.stderr, flush=True)
+        cpu = cpu_baseline_3d(args.n3d, host["threads"])
+        extra["3d_ve"]["gpu_over_cpu"] = round(r3["T_eff_GBs"] / cpu["value"], 1)
+        print(f"# CPU 3D: {cpu['value']:.2f} GB/s ({cpu['sample']})", file=sys.stderr, flush=True)
+        if not args.no_2d:
+            c2 = cpu_baseline_2d(args.n2d, host["threads"])
+            extra["2d_ve"]["cpu_baseline"] = c2
+            extra["2d_ve"]["gpu_over_cpu"] = round(extra["2d_ve"]["T_eff_GBs"] / c2["value"], 1)
+            print(f"# CPU 2D: {c2['value']:.2f} GB/s ({c2['sample']})", file=sys.stderr, flush=True)
 
-    extra["2d_ve_contexts"] = dict(ms_per_it_by_context=r2["ms_per_it_by_context"],
-                                   placement_ms_by_context=r2["placement_ms_by_context"],
-                                   placement_s_by_context=r2["placement_s_by_context"])
-    t_eff = r2["T_eff_GBs"]
+    t_eff = r3["T_eff_GBs"]
     line = {
         "metric": METRIC,
         "value": round(t_eff, 2),
@@ -544,18 +615,18 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r2["ms_per_it"], 5),
+        "ms_per_step": round(r3["ms_per_it"], 5),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(t_eff / A100_2D_VE_8191_GBS, 3),
+        "vs_baseline": round(t_eff / A100_3D_VE_511_GBS, 3),
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"2D Maxwell visco-elastic Stokes PT iteration, nx=ny={args.n2d}, FP64, "
-                        "G=eta=1, dt=0.1, pure shear, 1e-3 inclusion r=1 (single-pass kernel)",
-            "A_eff_bytes_per_step": r2["A_eff_bytes"],
+            "workload": f"3D Maxwell visco-elastic Stokes PT iteration, nx=ny=nz={args.n3d}, FP64, G=1, dt=1, "
+                        "x-z pure shear, 1e-3 sphere r=1 (single-pass kernel k3_iter2)",
+            "A_eff_bytes_per_step": r3["A_eff_bytes"],
             "T_eff_frac_of_hbm_peak": round(t_eff / HBM_PEAK_GBS, 4),
-            "baseline": "A100 T_eff 932.8 GB/s, output/out_Stokes2D_ve_perf_ampere.txt:45",
+            "baseline": "A100 T_eff 732.2 GB/s, output/out_Stokes3D_ve_perf_ampere.txt:15",
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -564,7 +635,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(t_eff / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(f"k2_iter_{args.n2d}"),
+            "traffic": traffic3["bytes"] if traffic3 else None,
         },
         "cpu_baseline": cpu,
         "extra": extra,

@@ -100,7 +100,9 @@ int pts_get_stream(const pts_ctx* ctx, void** stream);
  * context. */
 int pts_device_bytes(const pts_ctx* ctx, int64_t* bytes);
 /* No counterpart in the reference (placement of this library's device
- * blocks).  On its first pts_pt_iterate / pts_pt_solve, a single-pass context
+ * blocks).  Ahead of its first long run -- a pts_pt_solve whose iter_max + 1,
+ * or a pts_pt_iterate whose niter, is at least PTS_PLACE_BUDGET_ITERS, or a
+ * pts_pt_iterate with niter = 0, which asks for it -- a single-pass context
  * whose field blocks total at least PTS_PLACE_MIN_MB (default 4096) times
  * real iterations on copies of its state in its own blocks and in up to
  * PTS_3D_PLACE_TRIES (default 5) / PTS_2D_PLACE_TRIES (default 31) fresh sets

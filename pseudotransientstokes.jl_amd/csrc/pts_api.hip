@@ -46,13 +46,20 @@ struct pts_ctx {
     int rows_per_block = -32;   // 2D single-pass: rows per block (0: one round of resident blocks; <0: that, capped)
     int variant_2d = 7;         // 2D single-pass code variant (prefetch, 8-wave strips)
     bool persist_2d = false;    // 2D single-pass: persistent waves looping over tiles
+    // 2D small grids: pts_pt_solve runs each chunk of iterations between checks
+    // as one cooperative launch with grid barriers (up to multi_max_cells cells)
+    bool multi_2d = true;
+    int multi_rows_2d = 2;              // rows per tile of that launch
+    int64_t multi_max_cells = (int64_t)1 << 20;
+    unsigned* d_bar = nullptr;          // barrier count, generation, error flag
+    int* h_bar_err = nullptr;           // pinned copy of the error flag
     int planes_per_block = -160; // 3D single-pass: z planes per block (<0: fewest block rounds, capped)
     int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
     int variant_3d = 1;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
     bool group_3d = true;       // 3D: state fields, partners and τ_o in grouped allocations (ensure_group)
     bool map_flat_3d = false;   // k3_iter2: tiles in the flat (chunk, tile) order
-    bool w64_3d = false;        // k3_iter2: 64-bit plane bases also where 32-bit offsets fit
+    bool w64_3d = true;         // k3_iter2: per-slot pointers also where 32-bit slot offsets fit
     // placement selection (place_fields): tries (3D, 2D), the least bytes of
     // field blocks for which it runs, a forced outcome for tests (-1: by time)
     int place_tries_3d = 5;
@@ -354,6 +361,9 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_ROWS_PER_BLOCK")) c->rows_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_2D_VARIANT")) c->variant_2d = std::atoi(e);
     if (const char* e = std::getenv("PTS_2D_PERSIST")) c->persist_2d = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_2D_MULTI")) c->multi_2d = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_2D_MULTI_ROWS")) c->multi_rows_2d = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("PTS_2D_MULTI_MAX_CELLS")) c->multi_max_cells = std::atoll(e);
     if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
@@ -377,6 +387,10 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (e == hipSuccess) e = hipMalloc((void**)&c->red_out, sizeof(double) * kRedVals);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_red, sizeof(double) * kRedVals, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void**)&c->d_flag, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&c->d_bar, 4 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(c->d_bar, 0, 4 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_bar_err, sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) *c->h_bar_err = 0;
     if (e != hipSuccess) {
         std::string m = std::string("HIP init: ") + hipGetErrorString(e);
         pts_destroy(c);
@@ -407,6 +421,8 @@ int pts_destroy(pts_ctx* c) {
     if (c->red_partials) (void)hipFree(c->red_partials);
     if (c->red_out) (void)hipFree(c->red_out);
     if (c->d_flag) (void)hipFree(c->d_flag);
+    if (c->d_bar) (void)hipFree(c->d_bar);
+    if (c->h_bar_err) (void)hipHostFree(c->h_bar_err);
     if (c->h_red) (void)hipHostFree(c->h_red);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -889,8 +905,56 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s0, int64_t n, bool stor
     return PTS_OK;
 }
 
+// 2D small grids: n iterations of the default kernel in one cooperative
+// launch (1: done; 0: not applicable, use the launches or the graph).
+static int run_iters_multi(pts_ctx* c, const StepScalars& s0, int64_t n) {
+    if (c->nd != 2 || !c->multi_2d || c->variant_2d != 7 || !use_mx(c) || n < 2 || n > (1 << 30)) return 0;
+    if ((int64_t)c->nx * c->ny > c->multi_max_cells) return 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const bool ve = c->g.ve != 0, mul = c->g.fd_mode == PTS_FD_MUL, bf = c->g.body_force != 0;
+    // The kernel reads every input from `in` and swaps in/out each iteration,
+    // so both sets must be complete: the same read-only fields, the state
+    // fields from cur and alt respectively.
+    const Fields f0 = F(c);
+    Fields f1 = f0;
+    int st[16];
+    const int ns = state_fields(c, st);
+    for (int q = 0; q < ns; ++q) f1.p[st[q]] = c->alt[st[q]];
+    std::vector<int> reads = {PTS_PT, PTS_VX, PTS_VY, PTS_TXX, PTS_TYY, PTS_TXY, PTS_MUS, PTS_MUSTAU};
+    if (ve) reads.insert(reads.end(), {PTS_TXX_O, PTS_TYY_O, PTS_TXY_O});
+    if (bf) reads.push_back(PTS_RHOG);
+    for (int f : reads)
+        if (!f0.p[f] || !f1.p[f]) return 0;
+    hipError_t e = launch2_iter_multi(c->stream, f0, f1, with_dp(c, s0), ve, mul, bf, c->multi_rows_2d, (int)n,
+                                      c->d_bar, (int*)(c->d_bar + 2));
+    if (e == hipErrorNotSupported || e == hipErrorCooperativeLaunchTooLarge) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("cooperative launch: ") + hipGetErrorString(e));
+    e = hipMemcpyAsync(c->h_bar_err, c->d_bar + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("barrier flag: ") + hipGetErrorString(e));
+    for (int64_t q = 0; q < n; ++q) swap_state(c);
+    return 1;
+}
+
+// After a synchronisation: a grid barrier of a cooperative launch that gave up.
+static int barrier_error(pts_ctx* c) {
+    if (c->h_bar_err && *c->h_bar_err) return fail(c, PTS_ERR_HIP, "grid barrier wait gave up (results invalid)");
+    return PTS_OK;
+}
+
 static int run_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_last_dv, bool use_graph) {
     if (n <= 0) return PTS_OK;
+    if (use_graph) {  // (2D only: store_last_dv concerns the 3D kernel pair)
+        const int m = run_iters_multi(c, s, n);
+        if (m < 0) return m;
+        if (m == 1) return PTS_OK;
+    }
     if (!use_graph || !single_pass(c) || n < 2) return enqueue_iters(c, s, n, store_last_dv);
     // Capture n single-pass iterations once per (n, parity) and replay; a
     // graph captured with other buffers or scalars is captured again.
@@ -1119,6 +1183,8 @@ int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
     else if (k == "PTS_ROWS_PER_BLOCK") c->rows_per_block = v;
     else if (k == "PTS_2D_VARIANT") c->variant_2d = v;
     else if (k == "PTS_2D_PERSIST") c->persist_2d = v != 0;
+    else if (k == "PTS_2D_MULTI") c->multi_2d = v != 0;
+    else if (k == "PTS_2D_MULTI_ROWS") c->multi_rows_2d = std::max(1, v);
     else return fail(c, PTS_ERR_ARG, "unknown knob " + k);
     // captured graphs hold the old launch configuration
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
@@ -1132,12 +1198,21 @@ int pts_placement_times(const pts_ctx* c, float* ms, int32_t n) {
     return PTS_OK;
 }
 
+// Placement selection costs up to place_budget_iters iterations' worth of
+// time, so it runs ahead of a call that plans at least that many iterations
+// (pts_pt_solve: iter_max + 1) or when asked for (pts_pt_iterate with
+// niter = 0); shorter calls leave it pending.
+static bool place_now(const pts_ctx* c, int64_t planned) {
+    return planned == 0 || planned >= c->place_budget_iters;
+}
+
 int pts_pt_iterate(pts_ctx* c, const pts_pt_params* p, int64_t niter) {
     if (!c || !p || niter < 0) return fail(c, PTS_ERR_ARG, "bad argument");
     if (int rc = setdev(c)) return rc;
     if (int rc = prepare_iter(c)) return rc;
     StepScalars s = scal(p->r, p->G, p->dt, p->dx, p->dy, c->nd == 3 ? p->dz : 1.0);
-    if (int rc = place_fields(c, s)) return rc;
+    if (place_now(c, niter))
+        if (int rc = place_fields(c, s)) return rc;
     return run_iters(c, s, niter, false, false);
 }
 
@@ -1165,7 +1240,8 @@ int pts_pt_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp,
     if (int rc = setdev(c)) return rc;
     if (int rc = prepare_iter(c)) return rc;
     StepScalars s = scal(p->r, p->G, p->dt, p->dx, p->dy, c->nd == 3 ? p->dz : 1.0);
-    if (int rc = place_fields(c, s)) return rc;
+    if (place_now(c, sp->iter_max + 1))
+        if (int rc = place_fields(c, s)) return rc;
     std::memset(res, 0, sizeof(*res));
     int64_t iter = 0;
     double err = 2.0 * sp->eps;
@@ -1179,6 +1255,7 @@ int pts_pt_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp,
         if (check) {
             pts_norms nr;
             if (int rc = live_norms(c, s, &nr)) return rc;
+            if (int rc = barrier_error(c)) return rc;
             err = pts_err_from_norms(&nr, c->nd, p->lx, sp->norm_kind, res->comp);
             res->nchecks += 1;
             if (err != err) {
@@ -1189,6 +1266,7 @@ int pts_pt_solve(pts_ctx* c, const pts_pt_params* p, const pts_solve_params* sp,
         }
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (int rc = barrier_error(c)) return rc;
     res->iters = iter;
     res->err = err;
     res->converged = err <= sp->eps ? 1 : 0;

@@ -181,6 +181,14 @@ hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 // stores, 3 non-temporal stores.
 hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
                         bool mul, bool bf, int rows_per_block, int variant, bool persist, bool dp, bool mx);
+// `niter` iterations of the default single-pass kernel (derived Gdτ, dτ_Rho and
+// Musτ) in one cooperative launch with a grid barrier between iterations
+// (bar: 2 unsigned, count and generation, zeroed once; *err set if a barrier
+// wait gave up).  in/out swap with the parity of the iteration, as
+// pts_pt_iterate's ping-pong does.  hipErrorNotSupported when the tiles of
+// `rows` rows do not fit one round of resident blocks.
+hipError_t launch2_iter_multi(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
+                              bool mul, bool bf, int rows, int niter, unsigned* bar, int* err);
 // 1 in *bad unless Musτ equals compute_maxloc! of μ followed by bc_x!, bc_y!
 // (2D; the precondition of the mx iteration kernels).  *bad must be zero.
 hipError_t launch_check_mustau2(hipStream_t s, const Fields& f, int* bad);
@@ -204,8 +212,9 @@ hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 // field_guard_elems in pts_api.hip); otherwise k3_iter runs.  mx: form Musτ
 // from μ (k3_iter2 without prefetch); map_flat: k3_iter2 takes the tiles in
 // the flat (chunk, tile) order instead of each XCD sweeping its own tiles;
-// w64: k3_iter2 forms addresses from 64-bit plane bases even where 32-bit
-// offsets suffice (it does so by itself where they do not, e.g. at 1023^3).
+// w64: k3_iter2 with derived parameters addresses each slot of a grouped
+// field through its own pointer also where 32-bit slot offsets would fit (it
+// must where they do not, e.g. at 1023^3; at 511^3 it is about 1 % faster).
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
                         bool mul, bool bf, int planes_per_block, int tile_rows, bool dp, int variant,
                         int64_t slot_bytes, bool mx, bool map_flat, bool w64);

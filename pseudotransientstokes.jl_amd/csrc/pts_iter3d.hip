@@ -1004,7 +1004,7 @@ hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, cons
                         bool map_flat, bool w64) {
     if (Lz == 0) Lz = 1;
     Fields3g g;
-    const int gm = (variant == 1 || variant == 2) ? group_fields(in, out, ve, dp, bf, slot_bytes, w64, g) : 0;
+    const int gm = (variant == 1 || variant == 2) ? group_fields(in, out, ve, dp, bf, slot_bytes, w64 && dp, g) : 0;
     // per-slot pointers are instantiated for the derived-parameter kernels
     if (gm == 1 || (gm == 2 && dp)) {
         const bool pf = variant == 2;

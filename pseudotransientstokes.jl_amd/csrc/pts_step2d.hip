@@ -301,22 +301,68 @@ __device__ __forceinline__ void st(double* p, double v) {
 // rows whose cross is not in the pipeline (first and last row of a chunk,
 // grid row 0), the boundary columns and the strip's two halo lanes fall
 // back to the loads.
-template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, int MX>
-__global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(const Fields in, const Fields out, const StepScalars c, int L,
-                                                   int nstrips, int ntiles) {
+// Grid-wide barrier of a cooperative launch (every block resident): bar[0]
+// counts the arrivals, bar[1] is the generation (it only grows, so a launch
+// starts from whatever value the previous one left).  The writes before the
+// barrier are released and the reads after it acquired at device scope (the
+// XCDs' L2 caches are not coherent with each other).  The wait is bounded:
+// a block that sees no new generation after about 2^22 polls sets *err and
+// goes on, so a broken barrier ends the kernel with wrong results and an
+// error the host reports, never a hang.
+__device__ __forceinline__ unsigned barrier_generation(const unsigned* bar) {
+    return __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ void grid_barrier(unsigned* bar, unsigned nblocks, unsigned& gen, int* err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned g = gen;
+        const unsigned a = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (a == nblocks - 1) {
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(bar + 1, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            for (int spins = 0; __hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g;) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1 << 22)) {
+                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        gen = g + 1u;
+    }
+    __syncthreads();
+}
+
+// MULTI: one cooperative launch runs `niter` iterations, the ping-pong
+// buffers swapping with the parity of the iteration, a grid barrier between
+// iterations (small grids, where a launch costs more than the iteration).
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, int MX, bool MULTI = false>
+__global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(const Fields f0, const Fields f1, const StepScalars c, int L,
+                                                   int nstrips, int ntiles, int niter, unsigned* bar, int* berr) {
     constexpr int W = 64 * NW - 2;  // output columns per strip
     constexpr int NXL = 7;          // values passed to the right neighbour wave
     __shared__ double sL[NW > 1 ? NW : 1][NXL], sR[NW > 1 ? NW : 1][2], sT[NW > 1 ? NW : 1];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, gl = threadIdx.x;
-    const int nx = in.nx, ny = in.ny, sx = in.sx;
+    const int nx = f0.nx, ny = f0.ny, sx = f0.sx;
+    const double dx = c.dx, dy = c.dy, _dx = c._dx, _dy = c._dy, r = c.r, Gdt = c.Gdt;
+    const int nb = gridDim.x;
+    unsigned gen = 0;
+    if constexpr (MULTI) {
+        if (threadIdx.x == 0) gen = barrier_generation(bar);
+    }
+
+    for (int it = 0; it < (MULTI ? niter : 1); ++it) {
+    const Fields& in = (MULTI && (it & 1)) ? f1 : f0;
+    const Fields& out = (MULTI && (it & 1)) ? f0 : f1;
     double* __restrict__ Pt1 = out.p[PTS_PT];
     double* __restrict__ txx1 = out.p[PTS_TXX];
     double* __restrict__ tyy1 = out.p[PTS_TYY];
     double* __restrict__ txy1 = out.p[PTS_TXY];
     double* __restrict__ Vx1 = out.p[PTS_VX];
     double* __restrict__ Vy1 = out.p[PTS_VY];
-    const double dx = c.dx, dy = c.dy, _dx = c._dx, _dy = c._dy, r = c.r, Gdt = c.Gdt;
-    const int nb = gridDim.x;
 
     for (int b0 = blockIdx.x; b0 < ntiles; b0 += nb) {
         int b = b0;
@@ -503,6 +549,10 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
             }
         }
     }
+    if constexpr (MULTI) {
+        if (it + 1 < niter) grid_barrier(bar, (unsigned)nb, gen, berr);
+    }
+    }
 }
 
 }  // namespace
@@ -540,14 +590,14 @@ hipError_t launch2_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 }
 
 // Resident blocks of one k2_iter instance on the whole device.
-template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, int MX>
+template <bool VE, bool MUL, bool BF, bool PF, bool NT, int NW, bool DP, int MX, bool MULTI = false>
 static int concurrency2() {
     static int cached = 0;
     if (!cached) {
         int dev = 0, cus = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k2_iter<VE, MUL, BF, PF, NT, NW, DP, MX>, 64 * NW, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k2_iter<VE, MUL, BF, PF, NT, NW, DP, MX, MULTI>, 64 * NW, 0);
         cached = std::max(1, cus) * std::max(1, per);
     }
     return cached;
@@ -566,7 +616,27 @@ static void go2v(hipStream_t s, const Fields& in, const Fields& out, const StepS
     }
     const int ntiles = ns * ((in.ny + L - 1) / L);
     const int grid = persist ? std::min(ntiles, conc) : ntiles;
-    k2_iter<VE, MUL, BF, PF, NT, NW, DP, MX><<<grid, 64 * NW, 0, s>>>(in, out, c, L, ns, ntiles);
+    k2_iter<VE, MUL, BF, PF, NT, NW, DP, MX><<<grid, 64 * NW, 0, s>>>(in, out, c, L, ns, ntiles, 1, nullptr, nullptr);
+}
+
+// One cooperative launch of `niter` iterations of the default single-pass
+// kernel (8-wave strips, Gdτ, dτ_Rho and Musτ derived), `rows` rows per tile;
+// hipErrorNotSupported when its tiles do not fit one round of resident blocks.
+template <bool VE, bool MUL, bool BF>
+static hipError_t go2multi(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, int rows,
+                           int niter, unsigned* bar, int* err) {
+    constexpr int NW = 8, W = 64 * NW - 2;
+    auto kern = k2_iter<VE, MUL, BF, true, false, NW, true, 2, true>;
+    const int ns = (in.nx + 1 + W - 1) / W;
+    const int conc = concurrency2<VE, MUL, BF, true, false, NW, true, 2, true>();
+    int L = std::max(1, rows);
+    const int ntiles = ns * ((in.ny + L - 1) / L);
+    if (ntiles > conc) return hipErrorNotSupported;
+    Fields a = in, b = out;
+    StepScalars cc = c;
+    int nss = ns, nt = ntiles, ni = niter;
+    void* args[] = {&a, &b, &cc, &L, &nss, &nt, &ni, &bar, &err};
+    return hipLaunchCooperativeKernel((const void*)kern, dim3(ntiles), dim3(64 * NW), args, 0, s);
 }
 
 // dp: derive Gdτ, dτ_Rho from Musτ (variants 1, 7 and 10; the others read
@@ -591,6 +661,20 @@ static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepSc
         case 9: go2v<VE, MUL, BF, true, false, 16>(s, in, out, c, L, persist); break;
         default: go2v<VE, MUL, BF, true, false>(s, in, out, c, L, persist); break;
     }
+}
+
+hipError_t launch2_iter_multi(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
+                              bool mul, bool bf, int rows, int niter, unsigned* bar, int* err) {
+    if (ve) {
+        if (mul) return bf ? go2multi<true, true, true>(s, in, out, c, rows, niter, bar, err)
+                           : go2multi<true, true, false>(s, in, out, c, rows, niter, bar, err);
+        return bf ? go2multi<true, false, true>(s, in, out, c, rows, niter, bar, err)
+                  : go2multi<true, false, false>(s, in, out, c, rows, niter, bar, err);
+    }
+    if (mul) return bf ? go2multi<false, true, true>(s, in, out, c, rows, niter, bar, err)
+                       : go2multi<false, true, false>(s, in, out, c, rows, niter, bar, err);
+    return bf ? go2multi<false, false, true>(s, in, out, c, rows, niter, bar, err)
+              : go2multi<false, false, false>(s, in, out, c, rows, niter, bar, err);
 }
 
 hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,

@@ -16,18 +16,26 @@ def test_more_than_one_gpu_is_refused():
     assert r.stdout.strip() == ""
 
 
-def test_traffic_table_matches_the_counter_sums():
-    # bytes = (FETCH_SIZE x 2 + WRITE_SIZE) KiB x 1024, the calibration of
-    # profiles/pt_iteration_kernels_hbm_traffic.md (k2_iter 8191^2: 2877982 and
-    # 3164464 KiB; k3_iter2 511^3: 11341818 and 10856465 KiB; the table rounds
-    # the median counter values to whole KiB)
+def test_traffic_records_come_from_the_request_counters(monkeypatch):
+    # profiles/hbm_traffic.json: per kernel and grid, the L2-to-memory read
+    # requests by size (TCC_EA0_RDREQ_128B/64B/32B) and write requests
+    # (TCC_EA0_WRREQ, of which TCC_EA0_WRREQ_64B are 64-byte and the rest
+    # 32-byte), medians over the timed dispatches; bytes = read + write
     sys.path.insert(0, ROOT)
     import bench
-    assert abs(bench.load_traffic("k2_iter_8191") - (2877982 * 2 + 3164464) * 1024) < 3 * 1024
-    assert abs(bench.load_traffic("k3_iter2_511") - (11341818 * 2 + 10856465) * 1024) < 3 * 1024
     with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
-        assert set(json.load(f)) >= {"k2_iter_8191", "k3_iter2_511"}
-    assert bench.load_traffic("no_such_kernel") is None
+        table = json.load(f)["kernels"]
+    assert {"k3_iter2_511", "k2_iter_8191"} <= set(table)
+    for key, rec in table.items():
+        assert rec["read_bytes"] == 128 * rec["rdreq_128b"] + 64 * rec["rdreq_64b"] + 32 * rec["rdreq_32b"], key
+        assert rec["write_bytes"] == 64 * rec["wrreq_64b"] + 32 * (rec["wrreq"] - rec["wrreq_64b"]), key
+        assert rec["bytes"] == rec["read_bytes"] + rec["write_bytes"], key
+        assert len(rec["sources_sha256"]) == 64 and rec["profile"].startswith("profiles/"), key
+    # a record taken with other kernel sources is not reported as this build's
+    monkeypatch.setattr(bench, "sources_sha256", lambda kernel: "0" * 64)
+    rec, note = bench.load_traffic("k3_iter2", 511)
+    assert rec is None and "stale" in note
+    assert bench.load_traffic("k3_iter2", 12345)[0] is None
 
 
 def test_inversion_gradient_leg_drives_chunked_forward_solves(monkeypatch):
@@ -54,10 +62,10 @@ def test_inversion_gradient_leg_drives_chunked_forward_solves(monkeypatch):
             pass
 
     class FakeInversion:
-        def __init__(self, n, npts, eps, adj_eps):
-            assert (npts, eps, adj_eps) == (256, 1e-10, 1e-10)
+        def __init__(self, n, npts, eps, adj_eps, norm_kind):
+            assert (npts, eps, adj_eps, norm_kind) == (256, 1e-10, 1e-10, 0)
             self.ctx, self.params, self.eps, self.nout, self.iter_max = FakeCtx(), None, eps, 200, 10 ** 7
-            self.model = type("M", (), {"p": dict(norm_kind=1)})()
+            self.model = type("M", (), {"p": dict(norm_kind=norm_kind)})()
             self.last, self.obs = {}, None
 
         def grf_logmu(self, seed):
@@ -86,4 +94,4 @@ def test_inversion_gradient_leg_drives_chunked_forward_solves(monkeypatch):
     assert out["adjoint_iters"] == 42 and out["J"] == 0.25 and out["grad_finite"]
     assert out["forward_err"] == 1e-11
     # each chunk runs exactly `chunk` iterations at the inversion's tolerance and check interval
-    assert len(calls) == 6 and all(c == (1e-10, 200, 999, 1) for c in calls)
+    assert len(calls) == 6 and all(c == (1e-10, 200, 999, 0) for c in calls)

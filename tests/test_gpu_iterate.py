@@ -144,6 +144,7 @@ def test_moved_blocks_keep_state_and_graphs(accept, dim, monkeypatch):
     tries = f"PTS_{dim}_PLACE_TRIES"
     monkeypatch.setenv("PTS_PLACE_MIN_MB", "0")
     monkeypatch.setenv("PTS_PLACE_ACCEPT", accept)
+    monkeypatch.setenv("PTS_PLACE_BUDGET_ITERS", "30")  # the 30-iteration solve below is a long run
     monkeypatch.setenv(tries, "3")
     if dim == "3D":
         case = ("Stokes3D_ve_perf", dict(nx=45, ny=26, nz=21))
@@ -282,16 +283,22 @@ VARIANTS = [
     ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_3D_W64": "1", "PTS_DERIVED_MUSTAU": "0"}),
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=80), {"PTS_3D_W64": "1", "PTS_PLANES_PER_BLOCK": "64"}),
     ("Stokes3D_ve_perf", dict(nx=5, ny=4, nz=3), {"PTS_3D_W64": "1"}),
+    # one pointer per field with 32-bit slot offsets (the per-slot form is the default)
+    ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=30), {"PTS_3D_W64": "0"}),
+    ("Stokes3D_ve_multixpu", dict(n=19, rhog_in=-1.0), {"PTS_3D_W64": "0"}),
+    ("Stokes3D_ve_perf", dict(nx=63, ny=11, nz=30, ve=False, rhog_in=-1.0), {"PTS_3D_W64": "0", "PTS_3D_VARIANT": "2"}),
     ("Stokes3D_ve_perf", dict(nx=125, ny=22, nz=30, ve=False), {"PTS_PLANES_PER_BLOCK": "5"}),
     # placement selection moved into every size: blocks moved on every try,
     # never, or when faster
     ("Stokes3D_ve_perf", dict(nx=70, ny=23, nz=12),
-     {"PTS_PLACE_MIN_MB": "0", "PTS_PLACE_ACCEPT": "1", "PTS_3D_PLACE_TRIES": "2"}),
-    ("Stokes3D_ve_multixpu", dict(n=15, rhog_in=-1.0), {"PTS_PLACE_MIN_MB": "0", "PTS_PLACE_ACCEPT": "0"}),
-    ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_PLACE_MIN_MB": "0"}),
-    ("Stokes2D_ve_perf", dict(nx=300, ny=45), {"PTS_PLACE_MIN_MB": "0", "PTS_2D_PLACE_TRIES": "2"}),
+     {"PTS_PLACE_MIN_MB": "0", "PTS_PLACE_ACCEPT": "1", "PTS_3D_PLACE_TRIES": "2", "PTS_PLACE_BUDGET_ITERS": "1"}),
+    ("Stokes3D_ve_multixpu", dict(n=15, rhog_in=-1.0),
+     {"PTS_PLACE_MIN_MB": "0", "PTS_PLACE_ACCEPT": "0", "PTS_PLACE_BUDGET_ITERS": "1"}),
+    ("Stokes3D_ve_perf", dict(nx=40, ny=23, nz=19), {"PTS_PLACE_MIN_MB": "0", "PTS_PLACE_BUDGET_ITERS": "1"}),
+    ("Stokes2D_ve_perf", dict(nx=300, ny=45),
+     {"PTS_PLACE_MIN_MB": "0", "PTS_2D_PLACE_TRIES": "2", "PTS_PLACE_BUDGET_ITERS": "1"}),
     ("Stokes2D", dict(nx=253, ny=40, rhog_in=-1.0),
-     {"PTS_PLACE_MIN_MB": "0", "PTS_2D_PLACE_TRIES": "2", "PTS_PLACE_ACCEPT": "1"}),
+     {"PTS_PLACE_MIN_MB": "0", "PTS_2D_PLACE_TRIES": "2", "PTS_PLACE_ACCEPT": "1", "PTS_PLACE_BUDGET_ITERS": "1"}),
 ]
 
 

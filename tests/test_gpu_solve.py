@@ -145,3 +145,30 @@ def test_c1_viscous_127_dispersion_optimum_bit_exact():
     assert r["iters"] == res["ittot"] == 10200
     assert compare(m, o, STATE) == []
     m.close()
+
+
+@pytest.mark.parametrize("preset,kw,nout", [("Stokes2D", dict(n=127, rhog_in=-1.0, ebg=0.0), 10),
+                                            ("Stokes2D_ve_perf", dict(nx=90, ny=61), 7),
+                                            ("Stokes2D_ve", dict(n=63, dt=0.1), 200)])
+@pytest.mark.parametrize("rows", ["1", "2", "5"])
+def test_cooperative_chunks_equal_single_launches(preset, kw, nout, rows, monkeypatch):
+    # small 2D grids: pts_pt_solve runs each chunk between checks as one
+    # cooperative launch with grid barriers; the result equals one launch per
+    # iteration bit for bit, across odd and even chunk lengths
+    monkeypatch.setenv("PTS_2D_MULTI_ROWS", rows)
+    monkeypatch.setenv("PTS_2D_MULTI", "1")
+    a = StokesModel(preset, nout=nout, **kw).setup()
+    monkeypatch.setenv("PTS_2D_MULTI", "0")
+    b = StokesModel(preset, nout=nout, **kw).setup()
+    if a.p["ve"]:
+        a.ctx.assign_tau()
+        b.ctx.assign_tau()
+    params = a.pt_params()
+    for it_max in (29, 40):
+        ra = a.ctx.pt_solve(params, 1e-300, nout, it_max, 1, use_graph=True)
+        rb = b.ctx.pt_solve(params, 1e-300, nout, it_max, 1, use_graph=True)
+        assert ra["iters"] == rb["iters"] and ra["err"] == rb["err"], (ra, rb)
+        for f in ("Pt", "Vx", "Vy", "txx", "tyy", "txy"):
+            assert np.array_equal(a.get(f), b.get(f)), (f, it_max)
+    a.close()
+    b.close()

@@ -116,3 +116,28 @@ def test_create_rejects_bad_fd_mode():
     h = ctypes.c_void_p()
     assert lib.pts_create(ctypes.byref(g), ctypes.byref(h)) == _lib.PTS_ERR_ARG
     assert "fd_mode" in lib.pts_last_error(None).decode()
+
+
+def test_julia_script_entry_points_use_the_binding():
+    # julia/scripts/*.jl replay the reference scripts' entry points (ENV knobs
+    # NX, NY(, NZ), USE_RETURN, DO_SAVE, DO_SAVE_VIZ; README.md:71-79) on
+    # PTStokesAMD; Julia is absent here, so check them as text: every
+    # PTStokesAMD function they call exists in the binding
+    jdir = os.path.join(ROOT, "pseudotransientstokes.jl_amd", "julia")
+    binding = open(os.path.join(jdir, "PTStokesAMD.jl")).read()
+    defined = set(re.findall(r"^(?:function\s+)?([A-Za-z_][\w!]*)\s*\(", binding, re.M))
+    defined |= set(re.findall(r"^const\s+(\w+)", binding, re.M)) | set(re.findall(r"^(?:mutable\s+)?struct\s+(\w+)", binding, re.M))
+    sdir = os.path.join(jdir, "scripts")
+    common = open(os.path.join(sdir, "common.jl")).read()
+    scripts = {f: open(os.path.join(sdir, f)).read() for f in sorted(os.listdir(sdir)) if f != "common.jl"}
+    assert {"Stokes2D.jl", "Stokes2D_ve.jl", "Stokes2D_ve_perf.jl", "Stokes3D_ve_perf.jl",
+            "Stokes3D_ve_multixpu.jl"} <= set(scripts)
+    for name, txt in list(scripts.items()) + [("common.jl", common)]:
+        used = set(re.findall(r"\bPT\.([A-Za-z_][\w!]*)", txt))
+        assert used <= defined, (name, used - defined)
+    for name, txt in scripts.items():
+        assert 'include(joinpath(@__DIR__, "common.jl"))' in txt, name
+        for k in ("NX", "NY", "USE_RETURN", "DO_SAVE", "DO_SAVE_VIZ"):
+            assert f'knob("{k}"' in txt, (name, k)
+        if "3D" in name:
+            assert 'knob("NZ"' in txt, name
