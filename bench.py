@@ -144,28 +144,50 @@ def bench_3d(n, steps, warmup, ev, contexts=1):
     return out
 
 
-def bench_tts(n, nout=500, rtol_inner=1e-4, use_graph=True):
+def bench_tts(n, nout=500, rtol_inner=1e-4, use_graph=True, chunk=10000, max_outer=20):
     """Time to ‖res‖∞ < 1e-8 on the 3D viscous random-viscosity problem
     (log10 μ a seeded Gaussian random field clipped to ±2, 1e-3 sphere with
     ρg_z = -1): FP64 PT against FP32 PT sweeps inside FP64 defect
     correction, both from the same initial state, with the per-field error of
-    the mixed-precision solution against the FP64 one."""
+    the mixed-precision solution against the FP64 one.  The FP64 loop runs in
+    chunks of `chunk` iterations (a multiple of nout, so its checks are those
+    of one loop) and the defect correction one correction per call (the
+    outer loop carries no state between corrections), each reporting its
+    progress."""
     import numpy as np
     from ptstokes import StokesModel
     names = ["Pt", "Vx", "Vy", "Vz", "txx", "tyy", "tzz", "txy", "txz", "tyz"]
     m = StokesModel("Stokes3D_grf", n=n, nout=nout).setup()
+    p = m.p
     m.ctx.sync()
     t0 = time.perf_counter()
-    r64 = m.step(use_graph=use_graph)
+    it64 = 0
+    while True:
+        r64 = m.ctx.pt_solve(m.pt_params(), p["eps"], nout, chunk - 1, p["norm_kind"], use_graph=use_graph)
+        it64 += r64["iters"]
+        print(f"# TTS {n}^3 FP64 PT: {it64} iterations, err {r64['err']:.3e}, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        if r64["converged"] or it64 >= p["iter_max"]:
+            break
     m.ctx.sync()
     t64 = time.perf_counter() - t0
-    print(f"# TTS {n}^3 FP64 PT: {t64:.2f} s, {r64['iters']} iterations", file=sys.stderr, flush=True)
     ref = {f: m.get(f) for f in names}
     m.close()
     m = StokesModel("Stokes3D_grf", n=n).setup()
     m.ctx.sync()
     t0 = time.perf_counter()
-    rdc = m.solve_dc(rtol_inner=rtol_inner, nout_inner=nout)
+    outer, sweeps, per_corr, rdc = 0, 0, [], None
+    while True:
+        rdc = m.ctx.dc_solve(m.pt_params(), p["eps"], p["norm_kind"], 1, rtol_inner, nout)
+        if rdc["outer"] == 0:
+            break
+        outer += 1
+        sweeps += rdc["inner_iters"]
+        per_corr.append(rdc["inner_iters"])
+        print(f"# TTS {n}^3 FP32+DC: correction {outer}, {rdc['inner_iters']} sweeps, err {rdc['err']:.3e}, "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        if rdc["converged"] or outer >= max_outer:
+            break
     m.ctx.sync()
     tdc = time.perf_counter() - t0
     rel = {}
@@ -179,9 +201,9 @@ def bench_tts(n, nout=500, rtol_inner=1e-4, use_graph=True):
     return dict(workload=f"3D viscous Stokes, log10 mu GRF (seed 1234, corr. 0.5, sigma 1, clip 2) + 1e-3 sphere "
                          f"with rhog_z=-1, mu smoothed by 20 passes (10 leave the PT iteration unstable on this "
                          f"contrast), nx=ny=nz={n}, stop ||res||_inf < 1e-8 (scaled max-norm)",
-                fp64_s=round(t64, 3), fp64_iters=r64["iters"], fp64_err=r64["err"],
-                dc_s=round(tdc, 3), dc_outer=rdc["outer"], dc_fp32_sweeps=rdc["inner_iters"], dc_err=rdc["err"],
-                dc_converged=rdc["converged"], time_ratio_dc_over_fp64=round(tdc / t64, 3),
+                fp64_s=round(t64, 3), fp64_iters=it64, fp64_err=r64["err"], fp64_converged=r64["converged"],
+                dc_s=round(tdc, 3), dc_outer=outer, dc_fp32_sweeps=sweeps, dc_sweeps_per_correction=per_corr,
+                dc_err=rdc["err"], dc_converged=rdc["converged"], time_ratio_dc_over_fp64=round(tdc / t64, 3),
                 dc_vs_fp64_max_rel_err=rel)
 
 
@@ -272,11 +294,22 @@ def bench_c5_grad(n, true_seed=4321, seed=1234, chunk=20000):
     t0 = time.perf_counter()
     fwd = forward("forward")
     t_fwd = time.perf_counter() - t0
-    _, J = inv.dJ_dVx()
+    g, J = inv.dJ_dVx()
     t0 = time.perf_counter()
-    grad = inv.gradient()
+    # the adjoint loop continues from the current λ, so chunks of a multiple of
+    # nout iterations check where one loop would
+    inv.ctx.adjoint_set_source(g, None)
+    adj = dict(iters=0)
+    while True:
+        r = inv.ctx.adjoint_solve(inv.params, inv.adj_eps, inv.nout, chunk)
+        adj = dict(iters=adj["iters"] + r["iters"], err=r["err"])
+        print(f"# C5 {n}^2 adjoint: {adj['iters']} iterations, change {r['err']:.3e}", file=sys.stderr, flush=True)
+        if r["converged"]:
+            break
+        if adj["iters"] >= inv.iter_max:
+            raise RuntimeError(f"C5 adjoint did not converge in {adj['iters']} iterations")
+    grad = inv.ctx.grad_logmu(inv.params)
     t_adj = time.perf_counter() - t0
-    adj = inv.last["adjoint"]
     print(f"# C5 {n}^2: adjoint + gradient {t_adj:.2f} s, {adj['iters']} iterations", file=sys.stderr, flush=True)
     out = dict(workload=f"2D viscous inversion, nx=ny={n}, ln mu GRF (seed {seed}; observations from seed "
                         f"{true_seed}), 256 top-boundary Vx points, forward PT to err (scaled RMS) < 1e-10, adjoint PT "

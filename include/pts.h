@@ -238,7 +238,11 @@ int pts_pt_solve(pts_ctx* ctx, const pts_pt_params* p, const pts_solve_params* s
 /* No counterpart in the reference (FP64 only).  Viscous grids, multiply form.
  * Each outer step projects τ = 2μ ε̇(V), evaluates the FP64 residual (the stop
  * test uses it), solves the correction with FP32 single-pass PT sweeps until
- * its residual dropped by rtol_inner, and adds it to P and V in FP64. */
+ * its residual dropped by rtol_inner, and adds it to P and V in FP64.  An
+ * inner solve also ends when 40 consecutive checks bring its residual no 1 %
+ * below the best value so far (the FP32 roundoff floor, which grows with the
+ * grid size, can lie above rtol_inner); the next correction then starts from
+ * the FP64 residual that remains. */
 typedef struct pts_dc_params {
     double eps;            /* FP64 stop: err <= eps */
     int32_t norm_kind;     /* 0: scaled RMS (reference err); 1: scaled max-norm */

@@ -54,6 +54,8 @@ struct pts_ctx {
     unsigned* d_bar = nullptr;          // barrier count, generation, error flag
     int* h_bar_err = nullptr;           // pinned copy of the error flag
     int planes_per_block = -160; // 3D single-pass: z planes per block (<0: fewest block rounds, capped)
+    int dc_planes_per_block = -24; // the same for the 3D FP32 correction sweep
+    bool dc_dp = true;           // 3D FP32 sweep: Gdτ, dτ_Rho from Musτ when they are derived from it
     int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
     int variant_3d = 1;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
@@ -364,6 +366,8 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_2D_MULTI")) c->multi_2d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_2D_MULTI_ROWS")) c->multi_rows_2d = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("PTS_2D_MULTI_MAX_CELLS")) c->multi_max_cells = std::atoll(e);
+    if (const char* e = std::getenv("PTS_DC_PLANES_PER_BLOCK")) c->dc_planes_per_block = std::atoi(e);
+    if (const char* e = std::getenv("PTS_DC_DP")) c->dc_dp = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
@@ -1185,6 +1189,8 @@ int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
     else if (k == "PTS_2D_PERSIST") c->persist_2d = v != 0;
     else if (k == "PTS_2D_MULTI") c->multi_2d = v != 0;
     else if (k == "PTS_2D_MULTI_ROWS") c->multi_rows_2d = std::max(1, v);
+    else if (k == "PTS_DC_PLANES_PER_BLOCK") c->dc_planes_per_block = v;
+    else if (k == "PTS_DC_DP") c->dc_dp = v != 0;
     else return fail(c, PTS_ERR_ARG, "unknown knob " + k);
     // captured graphs hold the old launch configuration
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
@@ -1294,6 +1300,10 @@ static Fields32 fields32_of(const pts_ctx* c, float* const* src) {
     return f;
 }
 
+// An FP32 correction solve whose residual makes no 1 % progress over this many
+// checks has reached its roundoff floor.
+static const int kDcStallChecks = 40;
+
 int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pts_dc_result* res) {
     if (!c || !p || !dp || !res || dp->nout_inner < 1 || dp->max_outer < 1) return fail(c, PTS_ERR_ARG, "bad argument");
     if (c->g.ve) return fail(c, PTS_ERR_STATE, "defect correction is implemented for viscous grids");
@@ -1306,7 +1316,11 @@ int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pt
         if (int rc = ensure32(c, &c->cur32[st[q]])) return rc;
         if (int rc = ensure32(c, &c->alt32[st[q]])) return rc;
     }
-    const int consts[] = {PTS_MUS, PTS_GDTAU, PTS_DTAU_RHO};
+    // the 3D sweep derives Gdτ and dτ_Rho from Musτ when the context's are
+    // derived from it (compute_iter_params!, no write since)
+    const bool dcdp = c->nd == 3 && c->dc_dp && use_dp(c);
+    const std::vector<int> consts = dcdp ? std::vector<int>{PTS_MUS, PTS_MUSTAU}
+                                         : std::vector<int>{PTS_MUS, PTS_GDTAU, PTS_DTAU_RHO};
     for (int f : consts) {
         if (int rc = ensure32(c, &c->cur32[f])) return rc;
         HIPCHK(c, launch_to_f32(c->stream, c->cur32[f], c->cur[f], c->nalloc));
@@ -1325,6 +1339,8 @@ int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pt
     s32._dx = (float)s._dx;
     s32._dy = (float)s._dy;
     s32._dz = (float)s._dz;
+    s32.k1 = (float)c->dp.ip_k1;
+    s32.kg = (float)(c->dp.ip_k2 / (c->dp.ip_k1 * c->dp.ip_r2));
     for (int outer = 0;; ++outer) {
         HIPCHK(c, launch_dc_project(c->stream, F(c), s, c->nd));
         pts_norms nr;
@@ -1373,11 +1389,13 @@ int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pt
         // the last correction only needs to bring err below eps (with margin)
         const double rtol = std::isfinite(err) ? std::max(dp->rtol_inner, 0.25 * dp->eps / err) : dp->rtol_inner;
         int64_t it = 0;
-        double err_in = err_in0;
-        while (err_in > rtol * err_in0 && it < dp->iter_max_inner) {
+        double err_in = err_in0, best = err_in0;
+        int stalled = 0;  // checks since the residual last fell 1 % below its best
+        while (err_in > rtol * err_in0 && it < dp->iter_max_inner && stalled < kDcStallChecks) {
             const int64_t n = std::min<int64_t>(dp->nout_inner, dp->iter_max_inner - it);
             for (int64_t q = 0; q < n; ++q) {
-                HIPCHK(c, launch_dc_iter(c->stream, fields32_of(c, c->cur32), fields32_of(c, c->alt32), s32, c->nd));
+                HIPCHK(c, launch_dc_iter(c->stream, fields32_of(c, c->cur32), fields32_of(c, c->alt32), s32, c->nd,
+                                         c->dc_planes_per_block, dcdp));
                 for (int m = 0; m < ns; ++m) std::swap(c->cur32[st[m]], c->alt32[st[m]]);
             }
             it += n;
@@ -1386,6 +1404,12 @@ int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pt
             HIPCHK(c, hipStreamSynchronize(c->stream));
             err_in = weighted(c->h_red, c->h_red + 4);
             if (err_in != err_in) return fail(c, PTS_ERR_NAN, "NaN in the FP32 correction solve");
+            if (err_in < 0.99 * best) {
+                best = err_in;
+                stalled = 0;
+            } else {
+                ++stalled;
+            }
         }
         res->inner_iters += it;
         HIPCHK(c, launch_dc_accumulate(c->stream, F(c), fields32_of(c, c->cur32), c->nd));

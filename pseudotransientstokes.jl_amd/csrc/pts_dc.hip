@@ -139,7 +139,10 @@ __global__ __launch_bounds__(64) void k2_dc(const Fields32 in, const Fields32 ou
 // marching in z), viscous, multiply form, with sources.
 // ---------------------------------------------------------------------------
 constexpr int BY3 = 12;
-template <bool E3>  // E3: some extent is 3 (the free-slip copies of put_bc)
+// E3: some extent is 3 (the free-slip copies of put_bc).  DP: Gdτ and dτ_Rho
+// from Musτ (one array read instead of two; they only set the pseudo-time
+// steps, so the correction's fixed point does not depend on their rounding).
+template <bool E3, bool DP>
 __global__ __launch_bounds__(64 * BY3, 6) void k3_dc(const Fields32 in, const Fields32 out, const StepScalars32 cs,
                                                   int Lz, int ntx, int nty) {
     __shared__ float sh[4][BY3][64];
@@ -155,6 +158,10 @@ __global__ __launch_bounds__(64 * BY3, 6) void k3_dc(const Fields32 in, const Fi
     const int ke = min(kb + Lz, nz);
     const bool own = lane >= 1 && lane <= 62 && w >= 1 && w <= BY3 - 2;
     const int wu = w > 0 ? w - 1 : 0, wd = w < BY3 - 1 ? w + 1 : BY3 - 1;
+    // Row roles: the top halo row only supplies Pt and τyy of the row above
+    // the tile (its Vy update), the bottom halo row τxy and τyz of the row
+    // below; each loads only the arrays those need.
+    const bool top = w == 0, bot = w == BY3 - 1, mid = !top && !bot;
     const bool cell = c >= 0 && c < nx && d >= 0 && d < ny;
     const bool cellU = c >= 0 && c < nx && d >= 1 && d <= ny;
     const bool vxv = c >= 0 && c <= nx && d >= 0 && d < ny;
@@ -178,7 +185,7 @@ __global__ __launch_bounds__(64 * BY3, 6) void k3_dc(const Fields32 in, const Fi
     const int k0 = max(kb - 1, 0), k1 = min(ke, nz - 1);
     const int o0 = c + sx * d + sxy * k0;
     float VzC = cell ? in.p[PTS_VZ][o0] : 0.f;
-    float VzUC = cellU ? in.p[PTS_VZ][o0 - sx] : 0.f;
+    float VzUC = cellU && !top ? in.p[PTS_VZ][o0 - sx] : 0.f;
     for (int k = k0; k <= k1; ++k) {
         const int o = c + sx * d + sxy * k;
         const bool kin2 = k >= 1 && k <= nz - 2;
@@ -187,40 +194,56 @@ __global__ __launch_bounds__(64 * BY3, 6) void k3_dc(const Fields32 in, const Fi
         float tx0 = 0.f, ty0 = 0.f, tz0 = 0.f, txy0 = 0.f, txz0 = 0.f, tyz0 = 0.f;
         float Vx0c = 0.f, Vy0c = 0.f, GdU = 0.f, MuU = 0.f, RhU = 0.f, VxU = 0.f, VzUN = 0.f, sVx = 0.f, sVy = 0.f;
         if (cell) {
-            Pt0 = in.p[PTS_PT][o];
-            Gd = in.p[PTS_GDTAU][o];
+            if constexpr (DP) {
+                const float M = in.p[PTS_MUSTAU][o];
+                Gd = cs.kg * M;
+                if (mid) Rh = cs.k1 / M;
+            } else {
+                Gd = in.p[PTS_GDTAU][o];
+                if (mid) Rh = in.p[PTS_DTAU_RHO][o];
+            }
             Mu = in.p[PTS_MUS][o];
-            Rh = in.p[PTS_DTAU_RHO][o];
-            Vy0n = in.p[PTS_VY][o + sx];
             VzN = in.p[PTS_VZ][o + sxy];
-            sP = in.p[PTS_DIVV][o];
-            sVz = in.p[PTS_RZ][o];
-            if (k >= 1) sVy = in.p[PTS_RY][o - sxy];
+            if (!bot) {
+                Pt0 = in.p[PTS_PT][o];
+                Vy0n = in.p[PTS_VY][o + sx];
+                sP = in.p[PTS_DIVV][o];
+            }
+            if (mid) {
+                sVz = in.p[PTS_RZ][o];
+                if (k >= 1) sVy = in.p[PTS_RY][o - sxy];
+            }
         }
-        if (cellU) {
-            GdU = in.p[PTS_GDTAU][o - sx];
+        if (cellU && !top) {
+            if constexpr (DP) {
+                const float M = in.p[PTS_MUSTAU][o - sx];
+                GdU = cs.kg * M;
+                if (mid) RhU = cs.k1 / M;
+            } else {
+                GdU = in.p[PTS_GDTAU][o - sx];
+                if (mid) RhU = in.p[PTS_DTAU_RHO][o - sx];
+            }
             MuU = in.p[PTS_MUS][o - sx];
-            RhU = in.p[PTS_DTAU_RHO][o - sx];
             VzUN = in.p[PTS_VZ][o - sx + sxy];
         }
         if (vxv) {
             Vx0c = in.p[PTS_VX][o];
-            if (k >= 1) sVx = in.p[PTS_RX][o - sxy];
+            if (mid && k >= 1) sVx = in.p[PTS_RX][o - sxy];
         }
-        if (vxU) VxU = in.p[PTS_VX][o - sx];
+        if (vxU && !top) VxU = in.p[PTS_VX][o - sx];
         if (vyv) {
             Vy0c = in.p[PTS_VY][o];
-            if (!cell && k >= 1) sVy = in.p[PTS_RY][o - sxy];
+            if (mid && !cell && k >= 1) sVy = in.p[PTS_RY][o - sxy];
         }
         const bool hxx = txx_cd && kin2, hyy = tyy_cd && kin2, hzz = tzz_cd;
         const bool hxy = txy_cd && kin2, hxz = txz_cd && kin1, hyz = tyz_cd && kin1;
         const int oe = o - 1 - sx - sxy;
-        if (hxx) tx0 = in.p[PTS_TXX][o - sx - sxy];
-        if (hyy) ty0 = in.p[PTS_TYY][o - 1 - sxy];
-        if (hzz) tz0 = in.p[PTS_TZZ][o - 1 - sx];
-        if (hxy) txy0 = in.p[PTS_TXY][oe];
-        if (hxz) txz0 = in.p[PTS_TXZ][oe];
-        if (hyz) tyz0 = in.p[PTS_TYZ][oe];
+        if (hxx && mid) tx0 = in.p[PTS_TXX][o - sx - sxy];
+        if (hyy && !bot) ty0 = in.p[PTS_TYY][o - 1 - sxy];
+        if (hzz && mid) tz0 = in.p[PTS_TZZ][o - 1 - sx];
+        if (hxy && !top) txy0 = in.p[PTS_TXY][oe];
+        if (hxz && mid) txz0 = in.p[PTS_TXZ][oe];
+        if (hyz && !top) tyz0 = in.p[PTS_TYZ][oe];
 
         const float VxR = fright(Vx0c);
         const float GdL = fleft(Gd), MuL = fleft(Mu), GdUL = fleft(GdU), MuUL = fleft(MuU);
@@ -549,7 +572,8 @@ static int march_length(int n, int columns, int conc, int cap) {
     return std::max(4, std::min((n + nch - 1) / nch, cap));
 }
 
-hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim) {
+hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim,
+                          int lz, bool dp) {
     static int conc2 = 0, conc3 = 0;
     if (ndim == 2) {
         const int ns = (in.nx + 1 + 61) / 62;
@@ -558,12 +582,15 @@ hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out
         k2_dc<<<nb, 64, 0, s>>>(in, out, c, L, ns);
     } else {
         const int ntx = (in.nx + 1 + 61) / 62, nty = (in.ny + 1 + BY3 - 3) / (BY3 - 2);
-        const int Lz = march_length(in.nz, ntx * nty, device_concurrency(k3_dc<false>, 64 * BY3, conc3), 24);
+        const int Lz =
+            planes_per_block(lz, in.nz, (int64_t)ntx * nty, device_concurrency(k3_dc<false, false>, 64 * BY3, conc3));
         const int nch = (in.nz + Lz - 1) / Lz;
-        if (in.nx == 3 || in.ny == 3 || in.nz == 3)
-            k3_dc<true><<<ntx * nty * nch, dim3(64, BY3), 0, s>>>(in, out, c, Lz, ntx, nty);
-        else
-            k3_dc<false><<<ntx * nty * nch, dim3(64, BY3), 0, s>>>(in, out, c, Lz, ntx, nty);
+        const dim3 grid(ntx * nty * nch), block(64, BY3);
+        const bool e3 = in.nx == 3 || in.ny == 3 || in.nz == 3;
+        if (e3 && dp) k3_dc<true, true><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty);
+        else if (e3) k3_dc<true, false><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty);
+        else if (dp) k3_dc<false, true><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty);
+        else k3_dc<false, false><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty);
     }
     return hipGetLastError();
 }

@@ -148,6 +148,10 @@ __device__ __forceinline__ void iter_params_of(double M, const StepScalars& c, d
 
 struct StepScalars32 {
     float r, _dx, _dy, _dz;
+    // derived pseudo-time steps (3D sweep with dp): dτ_Rho = k1 / Musτ,
+    // Gdτ = kg · Musτ (compute_iter_params!'s Vpdτ·max_l/Re / Musτ and
+    // Vpdτ² / dτ_Rho / (r+2), folded)
+    float k1, kg;
 };
 
 // ---- kernel launchers (defined in the .hip files) ---------------------------
@@ -220,7 +224,13 @@ hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, cons
                         int64_t slot_bytes, bool mx, bool map_flat, bool w64);
 
 // Mixed-precision defect correction (pts_dc.hip)
-hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim);
+// lz: planes per block of the 3D sweep (<0: fewest block rounds within [4, -lz])
+// dp: the 3D sweep reads Musτ and derives Gdτ, dτ_Rho from it (c.k1, c.kg)
+hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim,
+                          int lz, bool dp);
+// z planes per block: Lz > 0 as given, else the length within [4, -Lz] with the
+// fewest rounds of resident blocks × plane steps
+int planes_per_block(int Lz, int nz, int64_t ntile, int64_t conc);
 hipError_t launch_dc_project(hipStream_t s, const Fields& f, const StepScalars& c, int ndim);
 hipError_t launch_dc_sources(hipStream_t s, const Fields& f, const Fields32& g, const StepScalars& c, int ndim,
                              bool bf);

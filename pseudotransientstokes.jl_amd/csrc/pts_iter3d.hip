@@ -26,6 +26,29 @@
 
 namespace pts {
 
+// Planes per block (Lz > 0: Lz).  Lz < 0: the tiles of a z-chunk start
+// together and march to its end, so a launch takes ceil(blocks / resident)
+// rounds of about L + 3 plane steps (the redundant plane below the chunk and
+// the start-up of a block); take the L in [4, -Lz] with the fewest such steps
+// (ties: the longer march).  At 511^3 (468 tiles, 256 resident) that is 86
+// planes: 2808 blocks in 11 nearly full rounds.
+int planes_per_block(int Lz, int nz, int64_t ntile, int64_t conc) {
+    if (Lz > 0) return Lz;
+    const int lmax = std::max(1, std::min(-Lz, nz));
+    int L = lmax;
+    int64_t best = -1;
+    for (int l = lmax; l >= std::min(4, lmax); --l) {
+        const int64_t nch = (nz + l - 1) / l;
+        const int64_t cost = (ntile * nch + conc - 1) / conc * (l + 3);
+        if (best < 0 || cost < best) {
+            best = cost;
+            L = l;
+        }
+    }
+    return L;
+}
+
+
 namespace {
 
 template <bool VE, bool MUL, bool BF, int BY, bool DP>
@@ -308,28 +331,6 @@ __global__ __launch_bounds__(64 * BY) void k3_iter(const Fields in, const Fields
         tyy1PU = tyy1U;
         txy1PD = txy1D;
     }
-}
-
-// Planes per block (Lz > 0: Lz).  Lz < 0: the tiles of a z-chunk start
-// together and march to its end, so a launch takes ceil(blocks / resident)
-// rounds of about L + 3 plane steps (the redundant plane below the chunk and
-// the start-up of a block); take the L in [4, -Lz] with the fewest such steps
-// (ties: the longer march).  At 511^3 (468 tiles, 256 resident) that is 86
-// planes: 2808 blocks in 11 nearly full rounds.
-int planes_per_block(int Lz, int nz, int64_t ntile, int64_t conc) {
-    if (Lz > 0) return Lz;
-    const int lmax = std::max(1, std::min(-Lz, nz));
-    int L = lmax;
-    int64_t best = -1;
-    for (int l = lmax; l >= std::min(4, lmax); --l) {
-        const int64_t nch = (nz + l - 1) / l;
-        const int64_t cost = (ntile * nch + conc - 1) / conc * (l + 3);
-        if (best < 0 || cost < best) {
-            best = cost;
-            L = l;
-        }
-    }
-    return L;
 }
 
 // Resident blocks of one k3_iter instance on the whole device.

@@ -61,10 +61,25 @@ def test_inversion_gradient_leg_drives_chunked_forward_solves(monkeypatch):
         def sync(self):
             pass
 
+        def adjoint_set_source(self, g, h):
+            adj_calls.append("source")
+
+        def adjoint_solve(self, params, eps, nout, iter_max):
+            adj_calls.append((eps, nout, iter_max))
+            done = len(adj_calls) == 3
+            return dict(iters=42 if done else iter_max, err=1e-11 if done else 1e-4, converged=done)
+
+        def grad_logmu(self, params):
+            adj_calls.append("grad")
+            return np.ones((n, n))
+
+    adj_calls = []
+
     class FakeInversion:
         def __init__(self, n, npts, eps, adj_eps, norm_kind):
             assert (npts, eps, adj_eps, norm_kind) == (256, 1e-10, 1e-10, 0)
             self.ctx, self.params, self.eps, self.nout, self.iter_max = FakeCtx(), None, eps, 200, 10 ** 7
+            self.adj_eps = adj_eps
             self.model = type("M", (), {"p": dict(norm_kind=norm_kind)})()
             self.last, self.obs = {}, None
 
@@ -80,10 +95,6 @@ def test_inversion_gradient_leg_drives_chunked_forward_solves(monkeypatch):
         def dJ_dVx(self):
             return None, 0.25
 
-        def gradient(self):
-            self.last["adjoint"] = dict(iters=42, err=1e-11)
-            return np.ones((n, n))
-
         def close(self):
             pass
 
@@ -91,7 +102,72 @@ def test_inversion_gradient_leg_drives_chunked_forward_solves(monkeypatch):
     monkeypatch.setattr(adjoint, "Inversion", FakeInversion)
     out = bench.bench_c5_grad(n, chunk=1000)
     assert out["observe_iters"] == out["forward_iters"] == 2 * 1000 + 500
-    assert out["adjoint_iters"] == 42 and out["J"] == 0.25 and out["grad_finite"]
+    assert out["adjoint_iters"] == 1000 + 42 and out["J"] == 0.25 and out["grad_finite"]
+    # the adjoint: source set once, check-aligned chunks from the current λ, then the gradient
+    assert adj_calls == ["source", (1e-10, 200, 1000), (1e-10, 200, 1000), "grad"]
     assert out["forward_err"] == 1e-11
     # each chunk runs exactly `chunk` iterations at the inversion's tolerance and check interval
     assert len(calls) == 6 and all(c == (1e-10, 200, 999, 0) for c in calls)
+
+
+def test_tts_leg_drives_chunked_fp64_and_one_correction_per_call(monkeypatch):
+    # bench_tts's control flow with a stand-in model: the FP64 loop in
+    # check-aligned chunks until converged, the defect correction one
+    # correction per call until converged, per-field differences recorded
+    sys.path.insert(0, ROOT)
+    import numpy as np
+    import bench
+    import ptstokes
+
+    log = []
+
+    class FakeCtx:
+        def __init__(self):
+            self.dc = 0
+
+        def sync(self):
+            pass
+
+        def pt_solve(self, params, eps, nout, iter_max, norm_kind, use_graph=True):
+            log.append(("pt", eps, nout, iter_max, norm_kind))
+            done = sum(1 for e in log if e[0] == "pt") == 3
+            return dict(iters=iter_max + 1, err=5e-9 if done else 1e-3, converged=done)
+
+        def dc_solve(self, params, eps, norm_kind, max_outer, rtol_inner, nout_inner):
+            log.append(("dc", eps, norm_kind, max_outer, rtol_inner, nout_inner))
+            self.dc += 1
+            done = self.dc == 2
+            return dict(outer=1, converged=done, inner_iters=100 * self.dc, err=2e-9 if done else 1e-5,
+                        err_outer=[1.0, 1e-5])
+
+    class FakeModel:
+        made = 0
+
+        def __init__(self, preset, n, nout=500):
+            assert preset == "Stokes3D_grf"
+            FakeModel.made += 1
+            self.second = FakeModel.made == 2
+            self.ctx = FakeCtx()
+            self.p = dict(eps=1e-8, norm_kind=1, iter_max=10 ** 6)
+
+        def setup(self):
+            return self
+
+        def pt_params(self):
+            return None
+
+        def get(self, name):
+            a = np.arange(125, dtype=float).reshape(5, 5, 5) + 1.0
+            return a * (1.0 + 1e-12) if self.second else a
+
+        def close(self):
+            pass
+
+    monkeypatch.setattr(ptstokes, "StokesModel", FakeModel)
+    out = bench.bench_tts(7, nout=500, chunk=2000)
+    assert out["fp64_iters"] == 3 * 2000 and out["fp64_converged"] and out["fp64_err"] == 5e-9
+    assert [e for e in log if e[0] == "pt"] == [("pt", 1e-8, 500, 1999, 1)] * 3
+    assert out["dc_outer"] == 2 and out["dc_fp32_sweeps"] == 300 and out["dc_sweeps_per_correction"] == [100, 200]
+    assert [e for e in log if e[0] == "dc"] == [("dc", 1e-8, 1, 1, 1e-4, 500)] * 2
+    assert out["dc_converged"] and out["dc_err"] == 2e-9
+    assert 0 < out["dc_vs_fp64_max_rel_err"]["Vx"] < 2e-12 and out["dc_vs_fp64_max_rel_err"]["Pt"] < 1e-11
