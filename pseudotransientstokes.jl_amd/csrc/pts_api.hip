@@ -56,15 +56,17 @@ struct pts_ctx {
     int planes_per_block = -160; // 3D single-pass: z planes per block (<0: fewest block rounds, capped)
     int dc_planes_per_block = -24; // the same for the 3D FP32 correction sweep
     bool dc_dp = true;           // 3D FP32 sweep: Gdτ, dτ_Rho from Musτ when they are derived from it
+    bool dc_map_xcd = true;      // 3D FP32 sweep: each XCD sweeps its eighth of the tiles chunk by chunk
     int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
     int variant_3d = 1;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
     bool group_3d = true;       // 3D: state fields, partners and τ_o in grouped allocations (ensure_group)
     bool map_flat_3d = false;   // k3_iter2: tiles in the flat (chunk, tile) order
     bool w64_3d = true;         // k3_iter2: per-slot pointers also where 32-bit slot offsets fit
+    bool stream_3d = true;      // k3_iter2 (per-slot pointers, Musτ from μ): streaming loads of the lines only one tile reads
     // placement selection (place_fields): tries (3D, 2D), the least bytes of
     // field blocks for which it runs, a forced outcome for tests (-1: by time)
-    int place_tries_3d = 5;
+    int place_tries_3d = 31;
     int place_tries_2d = 31;
     int64_t place_budget_iters = 5000;  // stop trying after this many iterations' worth of time
     bool group_2d = false;      // 2D: state fields, partners and τ_o in grouped allocations
@@ -368,6 +370,7 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_2D_MULTI_MAX_CELLS")) c->multi_max_cells = std::atoll(e);
     if (const char* e = std::getenv("PTS_DC_PLANES_PER_BLOCK")) c->dc_planes_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_DC_DP")) c->dc_dp = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_DC_MAP_XCD")) c->dc_map_xcd = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
@@ -375,6 +378,7 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_3D_GROUPS")) c->group_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_MAP_FLAT")) c->map_flat_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_W64")) c->w64_3d = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_3D_STREAM")) c->stream_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_PLACE_TRIES")) c->place_tries_3d = std::min(31, std::max(0, std::atoi(e)));
     if (const char* e = std::getenv("PTS_2D_PLACE_TRIES")) c->place_tries_2d = std::min(31, std::max(0, std::atoi(e)));
     if (const char* e = std::getenv("PTS_2D_GROUPS")) c->group_2d = std::atoi(e) != 0;
@@ -893,7 +897,7 @@ static int enqueue_iters(pts_ctx* c, const StepScalars& s0, int64_t n, bool stor
             else
                 HIPCHK(c, launch3_iter(c->stream, F(c), fields_of(c, c->alt), s, ve, mul, bf, c->planes_per_block,
                                        c->tile_rows_3d, dp, c->variant_3d, c->slot_bytes, use_mx(c),
-                                       c->map_flat_3d, c->w64_3d));
+                                       c->map_flat_3d, c->w64_3d, c->stream_3d));
             swap_state(c);
         } else {
             if (store_last_dv && q == n - 1 && !mul) {
@@ -1184,6 +1188,7 @@ int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
     else if (k == "PTS_3D_VARIANT") c->variant_3d = v;
     else if (k == "PTS_3D_MAP_FLAT") c->map_flat_3d = v != 0;
     else if (k == "PTS_3D_W64") c->w64_3d = v != 0;
+    else if (k == "PTS_3D_STREAM") c->stream_3d = v != 0;
     else if (k == "PTS_ROWS_PER_BLOCK") c->rows_per_block = v;
     else if (k == "PTS_2D_VARIANT") c->variant_2d = v;
     else if (k == "PTS_2D_PERSIST") c->persist_2d = v != 0;
@@ -1191,6 +1196,7 @@ int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
     else if (k == "PTS_2D_MULTI_ROWS") c->multi_rows_2d = std::max(1, v);
     else if (k == "PTS_DC_PLANES_PER_BLOCK") c->dc_planes_per_block = v;
     else if (k == "PTS_DC_DP") c->dc_dp = v != 0;
+    else if (k == "PTS_DC_MAP_XCD") c->dc_map_xcd = v != 0;
     else return fail(c, PTS_ERR_ARG, "unknown knob " + k);
     // captured graphs hold the old launch configuration
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
@@ -1395,7 +1401,7 @@ int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pt
             const int64_t n = std::min<int64_t>(dp->nout_inner, dp->iter_max_inner - it);
             for (int64_t q = 0; q < n; ++q) {
                 HIPCHK(c, launch_dc_iter(c->stream, fields32_of(c, c->cur32), fields32_of(c, c->alt32), s32, c->nd,
-                                         c->dc_planes_per_block, dcdp));
+                                         c->dc_planes_per_block, dcdp, c->dc_map_xcd));
                 for (int m = 0; m < ns; ++m) std::swap(c->cur32[st[m]], c->alt32[st[m]]);
             }
             it += n;

@@ -142,13 +142,27 @@ constexpr int BY3 = 12;
 // E3: some extent is 3 (the free-slip copies of put_bc).  DP: Gdτ and dτ_Rho
 // from Musτ (one array read instead of two; they only set the pseudo-time
 // steps, so the correction's fixed point does not depend on their rounding).
+// nchunk > 0: each XCD sweeps its own eighth of the tiles chunk by chunk
+// (as k3_iter2: all XCDs on the same planes, different rows; 511^3:
+// 4.31 -> 4.01 ms per sweep against the flat (chunk, tile) order dealt to
+// the XCDs in contiguous runs, nchunk = 0).
 template <bool E3, bool DP>
 __global__ __launch_bounds__(64 * BY3, 6) void k3_dc(const Fields32 in, const Fields32 out, const StepScalars32 cs,
-                                                  int Lz, int ntx, int nty) {
+                                                  int Lz, int ntx, int nty, int nchunk) {
     __shared__ float sh[4][BY3][64];
-    const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int ntile = ntx * nty;
-    const int tile = b % ntile, chunk = b / ntile;
+    int tile, chunk;
+    if (nchunk > 0) {
+        const int xcd = blockIdx.x & 7, seq = blockIdx.x >> 3;
+        const int lo = (int)((int64_t)ntile * xcd / 8), cnt = (int)((int64_t)ntile * (xcd + 1) / 8) - lo;
+        if (seq >= cnt * nchunk) return;
+        tile = lo + seq % cnt;
+        chunk = seq / cnt;
+    } else {
+        const int b = xcd_tile(blockIdx.x, gridDim.x);
+        tile = b % ntile;
+        chunk = b / ntile;
+    }
     const int tx = tile % ntx, ty = tile / ntx;
     const int lane = threadIdx.x, w = threadIdx.y;
     const int c = tx * 62 - 1 + lane;
@@ -573,7 +587,7 @@ static int march_length(int n, int columns, int conc, int cap) {
 }
 
 hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim,
-                          int lz, bool dp) {
+                          int lz, bool dp, bool map_xcd) {
     static int conc2 = 0, conc3 = 0;
     if (ndim == 2) {
         const int ns = (in.nx + 1 + 61) / 62;
@@ -582,15 +596,25 @@ hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out
         k2_dc<<<nb, 64, 0, s>>>(in, out, c, L, ns);
     } else {
         const int ntx = (in.nx + 1 + 61) / 62, nty = (in.ny + 1 + BY3 - 3) / (BY3 - 2);
-        const int Lz =
-            planes_per_block(lz, in.nz, (int64_t)ntx * nty, device_concurrency(k3_dc<false, false>, 64 * BY3, conc3));
-        const int nch = (in.nz + Lz - 1) / Lz;
-        const dim3 grid(ntx * nty * nch), block(64, BY3);
+        const int64_t conc = device_concurrency(k3_dc<false, false>, 64 * BY3, conc3);
+        const dim3 block(64, BY3);
         const bool e3 = in.nx == 3 || in.ny == 3 || in.nz == 3;
-        if (e3 && dp) k3_dc<true, true><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty);
-        else if (e3) k3_dc<true, false><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty);
-        else if (dp) k3_dc<false, true><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty);
-        else k3_dc<false, false><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty);
+        int Lz, nchk;
+        dim3 grid;
+        if (map_xcd) {  // per XCD: ceil(tiles / 8) tiles per chunk on an eighth of the resident blocks
+            const int64_t per = ((int64_t)ntx * nty + 7) / 8;
+            Lz = planes_per_block(lz, in.nz, per, std::max<int64_t>(1, conc / 8));
+            nchk = (in.nz + Lz - 1) / Lz;
+            grid = dim3((unsigned)(8 * per * nchk));
+        } else {
+            Lz = planes_per_block(lz, in.nz, (int64_t)ntx * nty, conc);
+            grid = dim3((unsigned)(ntx * nty * ((in.nz + Lz - 1) / Lz)));
+            nchk = 0;
+        }
+        if (e3 && dp) k3_dc<true, true><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty, nchk);
+        else if (e3) k3_dc<true, false><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty, nchk);
+        else if (dp) k3_dc<false, true><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty, nchk);
+        else k3_dc<false, false><<<grid, block, 0, s>>>(in, out, c, Lz, ntx, nty, nchk);
     }
     return hipGetLastError();
 }

@@ -221,13 +221,14 @@ hipError_t launch3_compute_V_perf(hipStream_t s, const Fields& f, const StepScal
 // must where they do not, e.g. at 1023^3; at 511^3 it is about 1 % faster).
 hipError_t launch3_iter(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
                         bool mul, bool bf, int planes_per_block, int tile_rows, bool dp, int variant,
-                        int64_t slot_bytes, bool mx, bool map_flat, bool w64);
+                        int64_t slot_bytes, bool mx, bool map_flat, bool w64, bool stream_loads);
 
 // Mixed-precision defect correction (pts_dc.hip)
 // lz: planes per block of the 3D sweep (<0: fewest block rounds within [4, -lz])
 // dp: the 3D sweep reads Musτ and derives Gdτ, dτ_Rho from it (c.k1, c.kg)
+// map_xcd: each XCD sweeps its own eighth of the tiles chunk by chunk
 hipError_t launch_dc_iter(hipStream_t s, const Fields32& in, const Fields32& out, const StepScalars32& c, int ndim,
-                          int lz, bool dp);
+                          int lz, bool dp, bool map_xcd);
 // z planes per block: Lz > 0 as given, else the length within [4, -Lz] with the
 // fewest rounds of resident blocks × plane steps
 int planes_per_block(int Lz, int nz, int64_t ntile, int64_t conc);
