@@ -257,7 +257,7 @@ def bench_c2_solve(n=8191, nt=1):
     return out
 
 
-def bench_c5_grad(n, true_seed=4321, seed=1234, chunk=20000):
+def bench_c5_grad(n, true_seed=4323, seed=1234, chunk=20000):
     """Time to ∂J/∂ln μ of the 2D viscous inversion: ln μ a seeded Gaussian
     random field, J = ½Σ|Vx − Vx_obs|² at 256 seeded points on the top
     boundary, observations from a hidden ln μ (one forward solve).  Then a
@@ -265,7 +265,12 @@ def bench_c5_grad(n, true_seed=4321, seed=1234, chunk=20000):
     PT solve to a relative change of 1e-10, and the gradient kernel; each
     phase timed on its own.  The forward solves run
     in chunks of `chunk` iterations (a multiple of the check interval, so the
-    checks and the result are those of one loop) to report progress."""
+    checks and the result are those of one loop) to report progress; a solve
+    whose err grows 10^4-fold above its best is stopped as diverged.  The
+    hidden field's seed is 4323: at 2047^2 the PT iteration diverges for the
+    fields of seeds 4321 and 4322 (err falls to ~1e-2, then grows; at CFL
+    0.9/sqrt(2) down to 0.6/sqrt(2) and with 20 or 40 smoothing passes alike),
+    while it converges to the roundoff floor for seeds 1234 and 4323."""
     import numpy as np
     from ptstokes.adjoint import Inversion
     # the reference's err (scaled RMS): the scaled max-norm has a roundoff floor
@@ -274,13 +279,16 @@ def bench_c5_grad(n, true_seed=4321, seed=1234, chunk=20000):
     truth, m0 = inv.grf_logmu(true_seed), inv.grf_logmu(seed)
 
     def forward(label):
-        iters = 0
+        iters, best = 0, float("inf")
         while True:
             r = inv.ctx.pt_solve(inv.params, inv.eps, inv.nout, chunk - 1, inv.model.p["norm_kind"])
             iters += r["iters"]
             print(f"# C5 {n}^2 {label}: {iters} iterations, err {r['err']:.3e}", file=sys.stderr, flush=True)
             if r["converged"]:
                 return dict(iters=iters, err=r["err"])
+            best = min(best, r["err"])
+            if not r["err"] < 1e4 * best:
+                raise RuntimeError(f"C5 {label} diverged (err {r['err']:.3e} after {iters} iterations)")
             if iters >= inv.iter_max:
                 raise RuntimeError(f"C5 {label} did not converge in {iters} iterations")
 
@@ -320,6 +328,47 @@ def bench_c5_grad(n, true_seed=4321, seed=1234, chunk=20000):
                grad_finite=bool(np.all(np.isfinite(grad))), time_to_gradient_s=round(t_fwd + t_adj, 3))
     inv.close()
     return out
+
+
+def bench_c5_floor(n, seed=1234, chunk=20000, max_s=300.0, stall_chunks=6, **overrides):
+    """Where the FP64 forward solve of the 2D inversion stops converging: the
+    reference's err (scaled RMS) and the scaled max-norm of the same residuals
+    after every `chunk` iterations, until the max-norm has not fallen 5 %
+    below its best for `stall_chunks` chunks (its roundoff floor) or after
+    `max_s` seconds.  Each point re-evaluates the residuals (compute_Res!) of
+    the current state."""
+    from ptstokes.adjoint import Inversion
+    inv = Inversion(n=n, npts=256, eps=1e-30, norm_kind=0, **overrides)
+    inv.set_logmu(inv.grf_logmu(seed))
+    p = inv.params
+    dx, dy = inv.model.dx, inv.model.dy
+    lx = inv.model.p["lx"]
+    curve, best, since = [], float("inf"), 0
+    iters = 0
+    t0 = time.perf_counter()
+    while True:
+        r = inv.ctx.pt_solve(p, 1e-30, inv.nout, chunk - 1, 0)
+        iters += r["iters"]
+        inv.ctx.compute_Res_perf(1.0 / dx, 1.0 / dy)
+        nr = inv.ctx.norms_stored()
+        e_rms, _ = inv.ctx.err_from_norms(nr, lx, 0)
+        e_inf, _ = inv.ctx.err_from_norms(nr, lx, 1)
+        el = time.perf_counter() - t0
+        curve.append([iters, float(e_rms), float(e_inf)])
+        print(f"# C5 floor {n}^2: {iters} iterations, err {e_rms:.3e} (RMS) {e_inf:.3e} (max), {el:.1f} s",
+              file=sys.stderr, flush=True)
+        if e_inf < 0.95 * best:
+            best, since = e_inf, 0
+        else:
+            since += 1
+        if since >= stall_chunks or el > max_s or not e_inf < 1e6:
+            break
+    inv.close()
+    floor = since >= stall_chunks
+    return dict(workload=f"2D viscous inversion forward solve, nx=ny={n}, ln mu GRF (seed {seed}) {overrides}",
+                iterations=iters, seconds=round(time.perf_counter() - t0, 2), floor_reached=floor,
+                min_err_rms=min(c[1] for c in curve), min_err_max=min(c[2] for c in curve),
+                curve=curve if len(curve) <= 64 else curve[:: max(1, len(curve) // 64)] + [curve[-1]])
 
 
 def bench_c4_kernels(n, ev):
@@ -556,6 +605,9 @@ def main():
     ap.add_argument("--c4-kernels", type=int, default=0,
                     help="also time the FP64 iteration and FP32 sweep of the 3D random-viscosity problem at this n "
                          "(1023: ~225 GB of device memory)")
+    ap.add_argument("--c5-floor", type=int, nargs="*", default=[],
+                    help="also follow the 2D inversion's FP64 forward err (RMS and max-norm) to its floor at these n")
+    ap.add_argument("--c5-floor-s", type=float, default=300.0, help="time limit of each --c5-floor run (s)")
     ap.add_argument("--c5-grad", type=int, default=0,
                     help="also time the 2D inversion's forward solve, adjoint solve and gradient at this n")
     ap.add_argument("--c2-solve-steps", type=int, default=0,
@@ -617,6 +669,8 @@ def main():
                   file=sys.stderr, flush=True)
     if args.c4_kernels > 0:
         extra["c4_kernels"] = bench_c4_kernels(args.c4_kernels, ev)
+    if args.c5_floor:
+        extra["c5_forward_err_floor"] = {str(n): bench_c5_floor(n, max_s=args.c5_floor_s) for n in args.c5_floor}
     if args.c5_grad > 0:
         extra["c5_inversion_gradient"] = bench_c5_grad(args.c5_grad)
     if args.c2_solve_steps > 0:
