@@ -587,6 +587,68 @@ def load_traffic(kernel, n):
     return rec, rec.get("profile", "profiles/hbm_traffic.json")
 
 
+def record_traffic(rd_dir, wr_dir, sizes):
+    """Write profiles/hbm_traffic.json from the counter CSVs of two rocprofv3
+    --pmc runs of bench.py (TCC_EA0_RDREQ_sum, _128B, _64B, _32B; then
+    TCC_EA0_WRREQ_sum, _64B): per kernel, the medians over the dispatches of
+    its most dispatched instantiation (the one the timed region runs), with the
+    hash of the kernel's sources.  sizes: {"k3_iter2": n3d, "k2_iter": n2d}."""
+    import csv
+    import glob
+    import statistics
+
+    def medians(d):
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise FileNotFoundError(f"no counter_collection.csv under {d}")
+        per = {}
+        for f in files:
+            with open(f) as fh:
+                for r in csv.DictReader(fh):
+                    per.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], []).append(
+                        float(r["Counter_Value"]))
+        out = {}
+        for kern in sizes:
+            names = [k for k in per if f"{kern}<" in k]
+            if not names:
+                continue
+            name = max(names, key=lambda k: max(len(v) for v in per[k].values()))
+            out[kern] = {c: statistics.median(v) for c, v in per[name].items()}
+        return out
+
+    rd, wr = medians(rd_dir), medians(wr_dir)
+    p = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(p) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        table = {}
+    table["note"] = ("HBM bytes per launch of the single-pass kernels, medians over the dispatches of one bench run: "
+                     "read = 128*TCC_EA0_RDREQ_128B + 64*TCC_EA0_RDREQ_64B + 32*TCC_EA0_RDREQ_32B, write = "
+                     "64*TCC_EA0_WRREQ_64B + 32*(TCC_EA0_WRREQ - TCC_EA0_WRREQ_64B); sources_sha256: the kernel's "
+                     "source files the record was measured with (bench.py reports the record only for the same "
+                     "sources); written by bench.py --record-traffic")
+    kernels = table.setdefault("kernels", {})
+    for kern, n in sizes.items():
+        if kern not in rd or kern not in wr:
+            continue
+        r, w = rd[kern], wr[kern]
+        rec = dict(rdreq=int(r["TCC_EA0_RDREQ_sum"]), rdreq_128b=int(r["TCC_EA0_RDREQ_128B_sum"]),
+                   rdreq_64b=int(r["TCC_EA0_RDREQ_64B_sum"]), rdreq_32b=int(r["TCC_EA0_RDREQ_32B_sum"]),
+                   wrreq=int(w["TCC_EA0_WRREQ_sum"]), wrreq_64b=int(w["TCC_EA0_WRREQ_64B_sum"]))
+        rec["read_bytes"] = 128 * rec["rdreq_128b"] + 64 * rec["rdreq_64b"] + 32 * rec["rdreq_32b"]
+        rec["write_bytes"] = 64 * rec["wrreq_64b"] + 32 * (rec["wrreq"] - rec["wrreq_64b"])
+        rec["bytes"] = rec["read_bytes"] + rec["write_bytes"]
+        rec["sources_sha256"] = sources_sha256(kern)
+        rec["profile"] = "profiles/pt_iteration_kernels_hbm_traffic.md"
+        kernels[f"{kern}_{n}"] = rec
+        print(f"# {kern} n={n}: read {rec['read_bytes'] / 1e9:.3f} GB, write {rec['write_bytes'] / 1e9:.3f} GB",
+              file=sys.stderr)
+    with open(p, "w") as f:
+        json.dump(table, f, indent=1)
+        f.write("\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -614,7 +676,12 @@ def main():
                     help="also run that many converged physical steps of 2D VE 8191^2 (minutes each)")
     ap.add_argument("--tts-no-graph", action="store_true",
                     help="FP64 time-to-tolerance without HIP graphs (for tracing tools)")
+    ap.add_argument("--record-traffic", nargs=2, metavar=("RD_DIR", "WR_DIR"),
+                    help="write profiles/hbm_traffic.json from two rocprofv3 --pmc runs of bench.py and exit (no GPU)")
     args = ap.parse_args()
+    if args.record_traffic:
+        record_traffic(args.record_traffic[0], args.record_traffic[1], {"k3_iter2": args.n3d, "k2_iter": args.n2d})
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU path is a design, not a build, in this repository)", file=sys.stderr)
