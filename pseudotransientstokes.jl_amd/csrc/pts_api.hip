@@ -51,6 +51,13 @@ struct pts_ctx {
     bool multi_2d = true;
     int multi_rows_2d = 2;              // rows per tile of that launch
     int64_t multi_max_cells = (int64_t)1 << 20;
+    bool lds_2d = true;                 // 2D small grids: state in LDS for a check interval (k2_lds)
+    int lds_rows_2d = 0;                // rows per block of that launch (0: the default choice)
+    int lds_threads_2d = 1024;          // threads per block of that launch
+    double* d_lds_xb = nullptr;         // its exchange rows
+    unsigned long long* d_lds_flags = nullptr;
+    int lds_blocks = 0;                 // blocks the buffers are sized for
+    unsigned long long lds_gen = 0;     // the counters' value between launches
     unsigned* d_bar = nullptr;          // barrier count, generation, error flag
     int* h_bar_err = nullptr;           // pinned copy of the error flag
     int planes_per_block = -160; // 3D single-pass: z planes per block (<0: fewest block rounds, capped)
@@ -366,6 +373,9 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_2D_VARIANT")) c->variant_2d = std::atoi(e);
     if (const char* e = std::getenv("PTS_2D_PERSIST")) c->persist_2d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_2D_MULTI")) c->multi_2d = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_2D_LDS")) c->lds_2d = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_2D_LDS_ROWS")) c->lds_rows_2d = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("PTS_2D_LDS_THREADS")) c->lds_threads_2d = std::min(1024, std::max(64, std::atoi(e) / 64 * 64));
     if (const char* e = std::getenv("PTS_2D_MULTI_ROWS")) c->multi_rows_2d = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("PTS_2D_MULTI_MAX_CELLS")) c->multi_max_cells = std::atoll(e);
     if (const char* e = std::getenv("PTS_DC_PLANES_PER_BLOCK")) c->dc_planes_per_block = std::atoi(e);
@@ -429,6 +439,8 @@ int pts_destroy(pts_ctx* c) {
     if (c->red_partials) (void)hipFree(c->red_partials);
     if (c->red_out) (void)hipFree(c->red_out);
     if (c->d_flag) (void)hipFree(c->d_flag);
+    if (c->d_lds_xb) (void)hipFree(c->d_lds_xb);
+    if (c->d_lds_flags) (void)hipFree(c->d_lds_flags);
     if (c->d_bar) (void)hipFree(c->d_bar);
     if (c->h_bar_err) (void)hipHostFree(c->h_bar_err);
     if (c->h_red) (void)hipHostFree(c->h_red);
@@ -950,6 +962,81 @@ static int run_iters_multi(pts_ctx* c, const StepScalars& s0, int64_t n) {
     return 1;
 }
 
+// 2D small grids: n iterations with the state in LDS in one cooperative
+// launch (k2_lds; 1: done; 0: not applicable).  R rows per block: 4 (or the
+// nearest count that works), at least 2, at most what the LDS holds
+// (160 KiB per CU), such that row ny-1 starts no block (bc_y!(Vx) copies
+// row ny-2 into it), and few enough blocks to be resident at once.
+static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
+    if (c->nd != 2 || !c->lds_2d || !single_pass(c) || n < 2 || n > (1 << 30)) return 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const bool ve = c->g.ve != 0, mul = c->g.fd_mode == PTS_FD_MUL, bf = c->g.body_force != 0;
+    const bool dp = use_dp(c);
+    const size_t lds_max = 160 * 1024;
+    const size_t per_row = (size_t)lds2_fields(ve, bf) * c->sx * sizeof(double);  // a row of every held field
+    const int fit = (int)(lds_max / per_row) - 2;
+    int R = 0;
+    if (c->lds_rows_2d > 0) {  // as asked, or the next fewer that works
+        for (int r = std::min(fit, c->lds_rows_2d); r >= 2 && !R; --r)
+            if ((c->ny - 1) % r != 0) R = r;
+    } else {  // short blocks: more of them in parallel (127^2: 4 rows 6.2 us, 8 rows 7.5 us per iteration)
+        for (int r : {4, 3, 5, 6, 7, 8, 2})
+            if (r <= fit && (c->ny - 1) % r != 0) {
+                R = r;
+                break;
+            }
+    }
+    if (R < 2) return 0;
+    const int nb = (c->ny + 1 + R - 1) / R;
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (nb > cus) return 0;
+    const Fields f0 = F(c);
+    Fields f1 = f0;
+    int st[16];
+    const int ns = state_fields(c, st);
+    for (int q = 0; q < ns; ++q) f1.p[st[q]] = c->alt[st[q]];
+    std::vector<int> reads = {PTS_PT, PTS_VX, PTS_VY, PTS_TXX, PTS_TYY, PTS_TXY, PTS_MUS};
+    if (dp) reads.push_back(PTS_MUSTAU);
+    else reads.insert(reads.end(), {PTS_GDTAU, PTS_DTAU_RHO});
+    if (ve) reads.insert(reads.end(), {PTS_TXX_O, PTS_TYY_O, PTS_TXY_O});
+    if (bf) reads.push_back(PTS_RHOG);
+    for (int f : reads)
+        if (!f0.p[f]) return 0;
+    for (int q = 0; q < ns; ++q)
+        if (!f1.p[st[q]]) return 0;
+    if (nb > c->lds_blocks) {
+        if (c->d_lds_xb) (void)hipFree(c->d_lds_xb);
+        if (c->d_lds_flags) (void)hipFree(c->d_lds_flags);
+        c->d_lds_xb = nullptr;
+        c->d_lds_flags = nullptr;
+        c->lds_blocks = 0;
+        HIPCHK(c, hipMalloc((void**)&c->d_lds_xb, (size_t)2 * nb * 2 * 6 * c->sx * sizeof(double)));
+        HIPCHK(c, hipMalloc((void**)&c->d_lds_flags, (size_t)nb * sizeof(unsigned long long)));
+        HIPCHK(c, hipMemsetAsync(c->d_lds_flags, 0, (size_t)nb * sizeof(unsigned long long), c->stream));
+        c->lds_blocks = nb;
+        c->lds_gen = 0;
+    }
+    hipError_t e = launch2_iter_lds(c->stream, f0, f1, with_dp(c, s0), ve, mul, bf, dp, R, (int)n, c->d_lds_xb,
+                                    c->d_lds_flags, c->lds_gen, (int*)(c->d_bar + 2), c->lds_threads_2d);
+    if (e == hipErrorNotSupported || e == hipErrorCooperativeLaunchTooLarge || e == hipErrorInvalidValue) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("cooperative launch: ") + hipGetErrorString(e));
+    // blocks that no block waits on (the flags of a grid of one block) stay put
+    c->lds_gen += (unsigned long long)(n - 1);
+    e = hipMemcpyAsync(c->h_bar_err, c->d_bar + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("exchange flag: ") + hipGetErrorString(e));
+    for (int64_t q = 0; q < n; ++q) swap_state(c);
+    return 1;
+}
+
 // After a synchronisation: a grid barrier of a cooperative launch that gave up.
 static int barrier_error(pts_ctx* c) {
     if (c->h_bar_err && *c->h_bar_err) return fail(c, PTS_ERR_HIP, "grid barrier wait gave up (results invalid)");
@@ -959,6 +1046,9 @@ static int barrier_error(pts_ctx* c) {
 static int run_iters(pts_ctx* c, const StepScalars& s, int64_t n, bool store_last_dv, bool use_graph) {
     if (n <= 0) return PTS_OK;
     if (use_graph) {  // (2D only: store_last_dv concerns the 3D kernel pair)
+        const int l = run_iters_lds(c, s, n);
+        if (l < 0) return l;
+        if (l == 1) return PTS_OK;
         const int m = run_iters_multi(c, s, n);
         if (m < 0) return m;
         if (m == 1) return PTS_OK;
@@ -1193,6 +1283,9 @@ int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
     else if (k == "PTS_2D_VARIANT") c->variant_2d = v;
     else if (k == "PTS_2D_PERSIST") c->persist_2d = v != 0;
     else if (k == "PTS_2D_MULTI") c->multi_2d = v != 0;
+    else if (k == "PTS_2D_LDS") c->lds_2d = v != 0;
+    else if (k == "PTS_2D_LDS_ROWS") c->lds_rows_2d = std::max(0, v);
+    else if (k == "PTS_2D_LDS_THREADS") c->lds_threads_2d = std::min(1024, std::max(64, v / 64 * 64));
     else if (k == "PTS_2D_MULTI_ROWS") c->multi_rows_2d = std::max(1, v);
     else if (k == "PTS_DC_PLANES_PER_BLOCK") c->dc_planes_per_block = v;
     else if (k == "PTS_DC_DP") c->dc_dp = v != 0;

@@ -555,6 +555,230 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
     }
 }
 
+// ---------------------------------------------------------------------------
+// Small 2D grids: the whole state in LDS for a whole check interval.  Block b
+// holds rows [r0, r1) = [b·R, b·R + R) of every field, plus one halo row on
+// each side, and runs `niter` iterations on them: the updates of
+// compute_P!/compute_τ! (cells and vertices, in place), of compute_V! (faces,
+// in place) and the free-slip copies bc_y!(Vx), bc_x!(Vy), each a phase
+// between barriers.  After every iteration a block hands its first and last
+// own rows to its neighbours through a small global buffer (two parities)
+// and a per-block counter (release and acquire at device scope; the wait is
+// bounded and sets *err when it gives up), so only the halo rows move through
+// memory: the state is read once at the start and written once at the end
+// (to f1 when niter is odd, else to f0; in the divide form the velocities
+// before the last iteration go to the other one, as the ping-pong leaves
+// them for the err).  A cooperative launch keeps every block resident.
+// Bit-identical to the per-kernel sequence.
+// ---------------------------------------------------------------------------
+constexpr int kLdsState = 6;  // Pt, τxx, τyy, τxy, Vx, Vy
+__device__ __forceinline__ int lds_state_field(int q) {
+    return q == 0 ? PTS_PT : q == 1 ? PTS_TXX : q == 2 ? PTS_TYY : q == 3 ? PTS_TXY : q == 4 ? PTS_VX : PTS_VY;
+}
+
+// LDS slot q holds rows L0 .. L0+NL-1 of one field; element (i, J) is at
+// i + sx·(J - L0) of the slot (a local index: every pointer stays inside the
+// LDS array).  Slots 0-5: the state (lds_state_field order), 6 μ, 7 Gdτ,
+// 8 dτ_Rho, then ρg (BF) and τxx_o, τyy_o, τxy_o (VE).
+template <bool VE, bool MUL, bool BF, bool DP>
+__global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1, const StepScalars c, int R,
+                                               int niter, double* xb, unsigned long long* flags,
+                                               unsigned long long gen0, int* err) {
+    extern __shared__ double smem[];
+    const int nx = f0.nx, ny = f0.ny, sx = f0.sx;
+    const int nb = gridDim.x, b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int r0 = b * R, r1 = min(r0 + R, ny + 1);  // own rows (Vy has faces up to row ny)
+    const int L0 = r0 - 1, NL = R + 2;                // LDS rows L0 .. L0 + NL - 1
+    const int lb = L0 * sx;                           // global index of local index 0
+    const size_t slot = (size_t)NL * sx;
+    double* const Pt = smem;
+    double* const txx = smem + slot;
+    double* const tyy = smem + 2 * slot;
+    double* const txy = smem + 3 * slot;
+    double* const Vx = smem + 4 * slot;
+    double* const Vy = smem + 5 * slot;
+    double* const Mu = smem + 6 * slot;
+    double* const Gd = smem + 7 * slot;
+    double* const Rh = smem + 8 * slot;
+    double* const rg = smem + 9 * slot;                  // BF
+    double* const txxo = smem + (9 + BF) * slot;         // VE
+    double* const tyyo = smem + (10 + BF) * slot;
+    double* const txyo = smem + (11 + BF) * slot;
+    // the LDS tile as Fields, for the residual functions of the per-kernel path
+    Fields lf = f0;
+    lf.p[PTS_PT] = Pt;
+    lf.p[PTS_TXX] = txx;
+    lf.p[PTS_TYY] = tyy;
+    lf.p[PTS_TXY] = txy;
+    lf.p[PTS_RHOG] = rg;
+    const int W = nx + 1;  // columns of the widest field
+    const int ja = max(L0, 0), jz = min(L0 + NL - 1, ny);  // rows held
+
+    // load the tile (rows outside the grid are never read)
+    for (int e = tid; e < (jz - ja + 1) * W; e += nt) {
+        const int J = ja + e / W, i = e % W, o = i + sx * J, l = o - lb;
+#pragma unroll
+        for (int q = 0; q < kLdsState; ++q) smem[q * slot + l] = f0.p[lds_state_field(q)][o];
+        Mu[l] = f0.p[PTS_MUS][o];
+        if constexpr (DP) {
+            double rh, gd;
+            iter_params_of(f0.p[PTS_MUSTAU][o], c, rh, gd);
+            Rh[l] = rh;
+            Gd[l] = gd;
+        } else {
+            Gd[l] = f0.p[PTS_GDTAU][o];
+            Rh[l] = f0.p[PTS_DTAU_RHO][o];
+        }
+        if constexpr (BF) rg[l] = f0.p[PTS_RHOG][o];
+        if constexpr (VE) {
+            txxo[l] = f0.p[PTS_TXX_O][o];
+            tyyo[l] = f0.p[PTS_TYY_O][o];
+            txyo[l] = f0.p[PTS_TXY_O][o];
+        }
+    }
+    __syncthreads();
+
+    const int ca = max(L0, 0), cz = min(r1 - 1, ny - 1);      // cell rows of the P, τ phase
+    const int va = max(L0, 0), vz = min(r1 - 1, ny - 2);      // vertex rows
+    const int xa = max(r0, 1), xz = min(r1 - 1, ny - 2);      // Vx face rows updated
+    const int ya = max(r0, 1), yz = min(r1 - 1, ny - 1);      // Vy face rows updated
+    const size_t xrow = (size_t)kLdsState * sx;               // one published row
+    for (int it = 0; it < niter; ++it) {
+        if (it > 0) {
+            // the neighbours' rows of the previous iteration
+            if (tid == 0) {
+                for (int nbr = b - 1; nbr <= b + 1; nbr += 2) {
+                    if (nbr < 0 || nbr >= nb) continue;
+                    for (int spins = 0; __hip_atomic_load(flags + nbr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+                                        gen0 + (unsigned long long)it;) {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > (1 << 22)) {
+                            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            __syncthreads();
+            const double* xp = xb + (size_t)((it - 1) & 1) * nb * 2 * xrow;
+            for (int e = tid; e < 2 * W; e += nt) {
+                const int side = e / W, i = e % W;
+                const int nbr = side == 0 ? b - 1 : b + 1, J = side == 0 ? L0 : r1;
+                if (nbr < 0 || nbr >= nb || J > ny) continue;
+                // the upper neighbour's last own row, the lower neighbour's first
+                const double* src = xp + ((size_t)nbr * 2 + (side == 0 ? 1 : 0)) * xrow + i;
+                const int l = i + sx * (J - L0);
+#pragma unroll
+                for (int q = 0; q < kLdsState; ++q) smem[q * slot + l] = src[(size_t)q * sx];
+            }
+            __syncthreads();
+        }
+        if (!MUL && it + 1 == niter) {
+            // the divide-form err takes ∇V from the velocities before the
+            // last update, from the ping-pong partner: leave them there
+            const Fields& prev = (niter & 1) ? f0 : f1;
+            for (int e = tid; e < (r1 - r0) * W; e += nt) {
+                const int J = r0 + e / W, i = e % W, o = i + sx * J;
+                prev.p[PTS_VX][o] = Vx[o - lb];
+                prev.p[PTS_VY][o] = Vy[o - lb];
+            }
+        }
+        // compute_P!, compute_τ! (Stokes2D.jl:33-44, Stokes2D_ve.jl:42-55,
+        // Stokes2D_ve_perf.jl:47-53): cells and vertices, in place
+        const int ncell = max(cz - ca + 1, 0) * nx, nvert = max(vz - va + 1, 0) * (nx - 1);
+        for (int e = tid; e < ncell + nvert; e += nt) {
+            if (e < ncell) {
+                const int J = ca + e / nx, i = e % nx, o = i + sx * (J - L0);
+                const double g = Gd[o], m = Mu[o];
+                if constexpr (MUL) {
+                    const double ex = c._dx * (Vx[o + 1] - Vx[o]), ey = c._dy * (Vy[o + sx] - Vy[o]);
+                    Pt[o] = Pt[o] - c.r * g * (ex + ey);
+                    const double txo = VE ? txxo[o] : 0.0, tyo = VE ? tyyo[o] : 0.0;
+                    txx[o] = tau_upd<VE>(txx[o], txo, g, m, c.Gdt, ex);
+                    tyy[o] = tau_upd<VE>(tyy[o], tyo, g, m, c.Gdt, ey);
+                } else {
+                    const double d = (Vx[o + 1] - Vx[o]) / c.dx + (Vy[o + sx] - Vy[o]) / c.dy;
+                    Pt[o] = Pt[o] - c.r * g * d;
+                    const double ex = Vx[o + 1] - Vx[o], ey = Vy[o + sx] - Vy[o];
+                    if constexpr (VE) {
+                        txx[o] = tau_upd<true>(txx[o], txxo[o], g, m, c.Gdt, ex / c.dx);
+                        tyy[o] = tau_upd<true>(tyy[o], tyyo[o], g, m, c.Gdt, ey / c.dy);
+                    } else {
+                        txx[o] = tau_visc2d_div(txx[o], g, m, ex, c.dx);
+                        tyy[o] = tau_visc2d_div(tyy[o], g, m, ey, c.dy);
+                    }
+                }
+            } else {
+                const int e2 = e - ncell, J = va + e2 / (nx - 1), i = e2 % (nx - 1), o = i + sx * (J - L0);
+                const double ag = (Gd[o] + Gd[o + 1] + Gd[o + sx] + Gd[o + sx + 1]) * 0.25;
+                const double am = (Mu[o] + Mu[o + 1] + Mu[o + sx] + Mu[o + sx + 1]) * 0.25;
+                const double ev = 0.5 * (dfd<MUL>(Vx[o + sx + 1] - Vx[o + 1], c.dy, c._dy) +
+                                         dfd<MUL>(Vy[o + sx + 1] - Vy[o + sx], c.dx, c._dx));
+                const double to = VE ? txyo[o] : 0.0;
+                txy[o] = tau_upd<VE>(txy[o], to, ag, am, c.Gdt, ev);
+            }
+        }
+        __syncthreads();
+        // compute_dV!, compute_V! (Stokes2D.jl:46-58, Stokes2D_ve_perf.jl:55-59),
+        // in place: the residual at index (i-1, J-1) updates face (i, J)
+        const int nvx = max(xz - xa + 1, 0) * (nx - 1), nvy = max(yz - ya + 1, 0) * (nx - 2);
+        for (int e = tid; e < nvx + nvy; e += nt) {
+            if (e < nvx) {
+                const int J = xa + e / (nx - 1), i = 1 + e % (nx - 1), o = (i - 1) + sx * (J - 1 - L0);
+                const double Rr = res2_x<MUL>(lf, c, o);
+                Vx[o + sx + 1] = Vx[o + sx + 1] + Rr * ((Rh[o + sx] + Rh[o + sx + 1]) * 0.5);
+            } else {
+                const int e2 = e - nvx, J = ya + e2 / (nx - 2), i = 1 + e2 % (nx - 2), o = (i - 1) + sx * (J - 1 - L0);
+                const double Rr = res2_y<MUL>(lf, c, o, BF);
+                Vy[o + sx + 1] = Vy[o + sx + 1] + Rr * ((Rh[o + 1] + Rh[o + sx + 1]) * 0.5);
+            }
+        }
+        __syncthreads();
+        // bc_y!(Vx), bc_x!(Vy) (Stokes2D_ve.jl:75-85): rows 0 and ny-1 of Vx
+        // (this block holds the row it copies: R >= 2 and ny-1 starts no block),
+        // columns 0 and nx-1 of Vy on the own rows
+        for (int e = tid; e < 2 * W + 2 * R; e += nt) {
+            if (e < 2 * W) {
+                const int i = e % W, J = e < W ? 0 : ny - 1, Js = e < W ? 1 : ny - 2;
+                if (J >= r0 && J < r1) Vx[i + sx * (J - L0)] = Vx[i + sx * (Js - L0)];
+            } else {
+                const int e2 = e - 2 * W, J = r0 + e2 / 2, right = e2 & 1;
+                if (J < r1) {
+                    const int l = sx * (J - L0);
+                    if (right) Vy[nx - 1 + l] = Vy[nx - 2 + l];
+                    else Vy[l] = Vy[1 + l];
+                }
+            }
+        }
+        __syncthreads();
+        if (it + 1 < niter) {
+            // hand the first and last own rows to the neighbours
+            double* xp = xb + (size_t)(it & 1) * nb * 2 * xrow + (size_t)b * 2 * xrow;
+            for (int e = tid; e < 2 * W; e += nt) {
+                const int side = e / W, i = e % W;
+                const int l = i + sx * ((side == 0 ? r0 : r1 - 1) - L0);
+                double* dst = xp + (size_t)side * xrow + i;
+#pragma unroll
+                for (int q = 0; q < kLdsState; ++q) dst[(size_t)q * sx] = smem[q * slot + l];
+            }
+            __syncthreads();
+            if (tid == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(flags + b, gen0 + (unsigned long long)it + 1ull, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    // the own rows of the state to the buffer the iterations end in
+    const Fields& out = (niter & 1) ? f1 : f0;
+    for (int e = tid; e < (r1 - r0) * W; e += nt) {
+        const int J = r0 + e / W, i = e % W, o = i + sx * J;
+#pragma unroll
+        for (int q = 0; q < kLdsState; ++q) out.p[lds_state_field(q)][o] = smem[q * slot + (o - lb)];
+    }
+}
+
 }  // namespace
 
 hipError_t launch2_compute_P(hipStream_t s, const Fields& f, const StepScalars& c) {
@@ -661,6 +885,43 @@ static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepSc
         case 9: go2v<VE, MUL, BF, true, false, 16>(s, in, out, c, L, persist); break;
         default: go2v<VE, MUL, BF, true, false>(s, in, out, c, L, persist); break;
     }
+}
+
+// Fields held in LDS per row by k2_lds, and its LDS bytes for R own rows.
+int lds2_fields(bool ve, bool bf) { return 9 + (bf ? 1 : 0) + (ve ? 3 : 0); }
+size_t lds2_bytes(int sx, int R, bool ve, bool bf) { return (size_t)lds2_fields(ve, bf) * (R + 2) * sx * sizeof(double); }
+
+template <bool VE, bool MUL, bool BF, bool DP>
+static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, int R, int niter,
+                         double* xb, unsigned long long* flags, unsigned long long gen0, int* err, int threads) {
+    auto kern = k2_lds<VE, MUL, BF, DP>;
+    const size_t bytes = lds2_bytes(f0.sx, R, VE, BF);
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    const int nb = (f0.ny + 1 + R - 1) / R;
+    Fields a = f0, bb = f1;
+    StepScalars cc = c;
+    int rr = R, ni = niter;
+    unsigned long long g0 = gen0;
+    void* args[] = {&a, &bb, &cc, &rr, &ni, &xb, &flags, &g0, &err};
+    return hipLaunchCooperativeKernel((const void*)kern, dim3(nb), dim3(threads), args, (unsigned)bytes, s);
+}
+
+hipError_t launch2_iter_lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, bool ve,
+                            bool mul, bool bf, bool dp, int R, int niter, double* xb, unsigned long long* flags,
+                            unsigned long long gen0, int* err, int threads) {
+#define PTS_LDS_GO(V, M, B, D) return go2lds<V, M, B, D>(s, f0, f1, c, R, niter, xb, flags, gen0, err, threads)
+    if (ve) {
+        if (mul) { if (bf) { if (dp) PTS_LDS_GO(true, true, true, true); PTS_LDS_GO(true, true, true, false); }
+                   if (dp) PTS_LDS_GO(true, true, false, true); PTS_LDS_GO(true, true, false, false); }
+        if (bf) { if (dp) PTS_LDS_GO(true, false, true, true); PTS_LDS_GO(true, false, true, false); }
+        if (dp) PTS_LDS_GO(true, false, false, true); PTS_LDS_GO(true, false, false, false);
+    }
+    if (mul) { if (bf) { if (dp) PTS_LDS_GO(false, true, true, true); PTS_LDS_GO(false, true, true, false); }
+               if (dp) PTS_LDS_GO(false, true, false, true); PTS_LDS_GO(false, true, false, false); }
+    if (bf) { if (dp) PTS_LDS_GO(false, false, true, true); PTS_LDS_GO(false, false, true, false); }
+    if (dp) PTS_LDS_GO(false, false, false, true); PTS_LDS_GO(false, false, false, false);
+#undef PTS_LDS_GO
 }
 
 hipError_t launch2_iter_multi(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,

@@ -155,6 +155,7 @@ def test_cooperative_chunks_equal_single_launches(preset, kw, nout, rows, monkey
     # small 2D grids: pts_pt_solve runs each chunk between checks as one
     # cooperative launch with grid barriers; the result equals one launch per
     # iteration bit for bit, across odd and even chunk lengths
+    monkeypatch.setenv("PTS_2D_LDS", "0")  # (the state-in-LDS launch comes first otherwise)
     monkeypatch.setenv("PTS_2D_MULTI_ROWS", rows)
     monkeypatch.setenv("PTS_2D_MULTI", "1")
     a = StokesModel(preset, nout=nout, **kw).setup()
@@ -165,6 +166,44 @@ def test_cooperative_chunks_equal_single_launches(preset, kw, nout, rows, monkey
         b.ctx.assign_tau()
     params = a.pt_params()
     for it_max in (29, 40):
+        ra = a.ctx.pt_solve(params, 1e-300, nout, it_max, 1, use_graph=True)
+        rb = b.ctx.pt_solve(params, 1e-300, nout, it_max, 1, use_graph=True)
+        assert ra["iters"] == rb["iters"] and ra["err"] == rb["err"], (ra, rb)
+        for f in ("Pt", "Vx", "Vy", "txx", "tyy", "txy"):
+            assert np.array_equal(a.get(f), b.get(f)), (f, it_max)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("preset,kw,nout,env", [
+    ("Stokes2D", dict(n=127, rhog_in=-1.0, ebg=0.0), 10, {}),
+    ("Stokes2D", dict(nx=70, ny=42, rhog_in=-1.0), 9, {"PTS_DERIVED_PARAMS": "0"}),
+    ("Stokes2D_ve_perf", dict(nx=90, ny=61), 7, {}),
+    ("Stokes2D_ve_perf", dict(nx=33, ny=100, rhog_in=-1.0), 12, {"PTS_DERIVED_PARAMS": "0"}),
+    ("Stokes2D_ve", dict(n=63, dt=0.1), 200, {}),
+    ("Stokes2D_ve", dict(nx=6, ny=5, rad2=9.0, rhog_in=-1.0), 3, {}),
+])
+@pytest.mark.parametrize("rows", ["0", "2", "7", "40"])
+def test_state_in_lds_chunks_equal_single_launches(preset, kw, nout, env, rows, monkeypatch):
+    # small 2D grids: each chunk between checks runs with the state in LDS,
+    # blocks of `rows` rows (0: as many as fit; fewer when row ny-1 would
+    # start a block) exchanging their edge rows every iteration; the result
+    # equals one launch per iteration bit for bit, across odd and even chunk
+    # lengths and repeated launches, over enough iterations for the flow to
+    # reach the walls (3 * nout + 1 after 69)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    monkeypatch.setenv("PTS_2D_LDS_ROWS", rows)
+    monkeypatch.setenv("PTS_2D_LDS", "1")
+    a = StokesModel(preset, nout=nout, **kw).setup()
+    monkeypatch.setenv("PTS_2D_LDS", "0")
+    monkeypatch.setenv("PTS_2D_MULTI", "0")
+    b = StokesModel(preset, nout=nout, **kw).setup()
+    if a.p["ve"]:
+        a.ctx.assign_tau()
+        b.ctx.assign_tau()
+    params = a.pt_params()
+    for it_max in (29, 40, 3 * nout + 1):
         ra = a.ctx.pt_solve(params, 1e-300, nout, it_max, 1, use_graph=True)
         rb = b.ctx.pt_solve(params, 1e-300, nout, it_max, 1, use_graph=True)
         assert ra["iters"] == rb["iters"] and ra["err"] == rb["err"], (ra, rb)
