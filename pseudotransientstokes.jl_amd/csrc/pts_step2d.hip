@@ -557,19 +557,21 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
 
 // ---------------------------------------------------------------------------
 // Small 2D grids: the whole state in LDS for a whole check interval.  Block b
-// holds rows [r0, r1) = [b·R, b·R + R) of every field, plus one halo row on
+// holds rows [r0, r1) = [b·R, b·R + R) of every field, plus H halo rows on
 // each side, and runs `niter` iterations on them: the updates of
-// compute_P!/compute_τ! (cells and vertices, in place), of compute_V! (faces,
-// in place) and the free-slip copies bc_y!(Vx), bc_x!(Vy), each a phase
-// between barriers.  After every iteration a block hands its first and last
-// own rows to its neighbours through a small global buffer (two parities)
-// and a per-block counter (release and acquire at device scope; the wait is
-// bounded and sets *err when it gives up), so only the halo rows move through
-// memory: the state is read once at the start and written once at the end
-// (to f1 when niter is odd, else to f0; in the divide form the velocities
-// before the last iteration go to the other one, as the ping-pong leaves
-// them for the err).  A cooperative launch keeps every block resident.
-// Bit-identical to the per-kernel sequence.
+// compute_P!/compute_τ! (cells and vertices, in place) and of compute_V!
+// (faces, in place, with the free-slip copies of bc_y!(Vx), bc_x!(Vy)), each
+// a phase between barriers.  Every H iterations a block hands its first and
+// last H own rows to its neighbours through a small global buffer (two
+// parities) and a per-block counter (release and acquire at device scope;
+// the wait is bounded and sets *err when it gives up); in between it also
+// updates the halo rows that are still valid (H - 1 - step rows beyond its
+// own on each side), so only the halo rows move through memory: the state
+// is read once at the start and written once at the end (to f1 when niter
+// is odd, else to f0; in the divide form the velocities before the last
+// iteration go to the other one, as the ping-pong leaves them for the err).
+// A cooperative launch keeps every block resident.  Bit-identical to the
+// per-kernel sequence.
 // ---------------------------------------------------------------------------
 constexpr int kLdsState = 6;  // Pt, τxx, τyy, τxy, Vx, Vy
 __device__ __forceinline__ int lds_state_field(int q) {
@@ -580,7 +582,7 @@ __device__ __forceinline__ int lds_state_field(int q) {
 // i + sx·(J - L0) of the slot (a local index: every pointer stays inside the
 // LDS array).  Slots 0-5: the state (lds_state_field order), 6 μ, 7 Gdτ,
 // 8 dτ_Rho, then ρg (BF) and τxx_o, τyy_o, τxy_o (VE).
-template <bool VE, bool MUL, bool BF, bool DP>
+template <bool VE, bool MUL, bool BF, bool DP, int H>
 __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1, const StepScalars c, int R,
                                                int niter, double* xb, unsigned long long* flags,
                                                unsigned long long gen0, int* err) {
@@ -588,7 +590,7 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
     const int nx = f0.nx, ny = f0.ny, sx = f0.sx;
     const int nb = gridDim.x, b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const int r0 = b * R, r1 = min(r0 + R, ny + 1);  // own rows (Vy has faces up to row ny)
-    const int L0 = r0 - 1, NL = R + 2;                // LDS rows L0 .. L0 + NL - 1
+    const int L0 = r0 - H, NL = R + 2 * H;            // LDS rows L0 .. L0 + NL - 1
     const int lb = L0 * sx;                           // global index of local index 0
     const size_t slot = (size_t)NL * sx;
     double* const Pt = smem;
@@ -638,19 +640,23 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
     }
     __syncthreads();
 
-    const int ca = max(L0, 0), cz = min(r1 - 1, ny - 1);      // cell rows of the P, τ phase
-    const int va = max(L0, 0), vz = min(r1 - 1, ny - 2);      // vertex rows
-    const int xa = max(r0, 1), xz = min(r1 - 1, ny - 2);      // Vx face rows updated
-    const int ya = max(r0, 1), yz = min(r1 - 1, ny - 1);      // Vy face rows updated
     const size_t xrow = (size_t)kLdsState * sx;               // one published row
     for (int it = 0; it < niter; ++it) {
-        if (it > 0) {
+        // step st of an exchange period of H iterations: the rows still valid
+        // shrink by one per step on each side (g rows beyond the own ones)
+        const int st = it % H, g = H - 1 - st;
+        const int ca = max(r0 - H + st, 0), cz = min(r1 - 1 + g, ny - 1);  // cell rows of the P, τ phase
+        const int va = ca, vz = min(r1 - 1 + g, ny - 2);                   // vertex rows
+        const int xa = max(r0 - g, 1), xz = min(r1 - 1 + g, ny - 2);       // Vx face rows updated
+        const int ya = max(r0 - g, 1), yz = min(r1 - 1 + g, ny - 1);       // Vy face rows updated
+        const int ba = max(r0 - g, 0), bz = min(r1 - 1 + g, ny);           // rows of the free-slip copies
+        if (it > 0 && st == 0) {
             // the neighbours' rows of the previous iteration
             if (tid == 0) {
                 for (int nbr = b - 1; nbr <= b + 1; nbr += 2) {
                     if (nbr < 0 || nbr >= nb) continue;
                     for (int spins = 0; __hip_atomic_load(flags + nbr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
-                                        gen0 + (unsigned long long)it;) {
+                                        gen0 + (unsigned long long)(it / H);) {
                         __builtin_amdgcn_s_sleep(1);
                         if (++spins > (1 << 22)) {
                             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -661,13 +667,13 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             __syncthreads();
-            const double* xp = xb + (size_t)((it - 1) & 1) * nb * 2 * xrow;
-            for (int e = tid; e < 2 * W; e += nt) {
-                const int side = e / W, i = e % W;
-                const int nbr = side == 0 ? b - 1 : b + 1, J = side == 0 ? L0 : r1;
+            const double* xp = xb + (size_t)((it / H - 1) & 1) * nb * 2 * H * xrow;
+            for (int e = tid; e < 2 * H * W; e += nt) {
+                const int side = e / (H * W), h = (e / W) % H, i = e % W;
+                const int nbr = side == 0 ? b - 1 : b + 1, J = side == 0 ? L0 + h : r1 + h;
                 if (nbr < 0 || nbr >= nb || J > ny) continue;
-                // the upper neighbour's last own row, the lower neighbour's first
-                const double* src = xp + ((size_t)nbr * 2 + (side == 0 ? 1 : 0)) * xrow + i;
+                // the upper neighbour's last H own rows, the lower neighbour's first H
+                const double* src = xp + (((size_t)nbr * 2 + (side == 0 ? 1 : 0)) * H + h) * xrow + i;
                 const int l = i + sx * (J - L0);
 #pragma unroll
                 for (int q = 0; q < kLdsState; ++q) smem[q * slot + l] = src[(size_t)q * sx];
@@ -721,51 +727,53 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
         }
         __syncthreads();
         // compute_dV!, compute_V! (Stokes2D.jl:46-58, Stokes2D_ve_perf.jl:55-59),
-        // in place: the residual at index (i-1, J-1) updates face (i, J)
+        // in place: the residual at index (i-1, J-1) updates face (i, J); the
+        // free-slip copies of bc_y!(Vx), bc_x!(Vy) (Stokes2D_ve.jl:75-85) are
+        // stored with it (this block updates the row it copies: R >= 2, and
+        // the updated rows never start at ny-1), and the copies of the
+        // Dirichlet corners (columns 0, nx of Vx; rows 0, ny of Vy) follow
         const int nvx = max(xz - xa + 1, 0) * (nx - 1), nvy = max(yz - ya + 1, 0) * (nx - 2);
-        for (int e = tid; e < nvx + nvy; e += nt) {
+        for (int e = tid; e < nvx + nvy + 8; e += nt) {
             if (e < nvx) {
                 const int J = xa + e / (nx - 1), i = 1 + e % (nx - 1), o = (i - 1) + sx * (J - 1 - L0);
                 const double Rr = res2_x<MUL>(lf, c, o);
-                Vx[o + sx + 1] = Vx[o + sx + 1] + Rr * ((Rh[o + sx] + Rh[o + sx + 1]) * 0.5);
-            } else {
+                const double v = Vx[o + sx + 1] + Rr * ((Rh[o + sx] + Rh[o + sx + 1]) * 0.5);
+                Vx[o + sx + 1] = v;
+                if (J == 1) Vx[o + 1] = v;                   // Vx[:, 0] = Vx[:, 1]
+                if (J == ny - 2) Vx[o + 2 * sx + 1] = v;     // Vx[:, ny-1] = Vx[:, ny-2]
+            } else if (e < nvx + nvy) {
                 const int e2 = e - nvx, J = ya + e2 / (nx - 2), i = 1 + e2 % (nx - 2), o = (i - 1) + sx * (J - 1 - L0);
                 const double Rr = res2_y<MUL>(lf, c, o, BF);
-                Vy[o + sx + 1] = Vy[o + sx + 1] + Rr * ((Rh[o + 1] + Rh[o + sx + 1]) * 0.5);
-            }
-        }
-        __syncthreads();
-        // bc_y!(Vx), bc_x!(Vy) (Stokes2D_ve.jl:75-85): rows 0 and ny-1 of Vx
-        // (this block holds the row it copies: R >= 2 and ny-1 starts no block),
-        // columns 0 and nx-1 of Vy on the own rows
-        for (int e = tid; e < 2 * W + 2 * R; e += nt) {
-            if (e < 2 * W) {
-                const int i = e % W, J = e < W ? 0 : ny - 1, Js = e < W ? 1 : ny - 2;
-                if (J >= r0 && J < r1) Vx[i + sx * (J - L0)] = Vx[i + sx * (Js - L0)];
+                const double v = Vy[o + sx + 1] + Rr * ((Rh[o + 1] + Rh[o + sx + 1]) * 0.5);
+                Vy[o + sx + 1] = v;
+                if (i == 1) Vy[o + sx] = v;                  // Vy[0, :] = Vy[1, :]
+                if (i == nx - 2) Vy[o + sx + 2] = v;         // Vy[nx-1, :] = Vy[nx-2, :]
             } else {
-                const int e2 = e - 2 * W, J = r0 + e2 / 2, right = e2 & 1;
-                if (J < r1) {
-                    const int l = sx * (J - L0);
-                    if (right) Vy[nx - 1 + l] = Vy[nx - 2 + l];
-                    else Vy[l] = Vy[1 + l];
+                const int k = e - nvx - nvy;
+                if (k < 4) {  // Vx at the Dirichlet columns 0 and nx, rows 0 and ny-1
+                    const int i = (k & 1) ? nx : 0, J = k < 2 ? 0 : ny - 1, Js = k < 2 ? 1 : ny - 2;
+                    if (J >= ba && J <= bz) Vx[i + sx * (J - L0)] = Vx[i + sx * (Js - L0)];
+                } else {      // Vy at the Dirichlet rows 0 and ny, columns 0 and nx-1
+                    const int J = (k & 1) ? ny : 0, i = k < 6 ? 0 : nx - 1, is = k < 6 ? 1 : nx - 2;
+                    if (J >= ba && J <= bz) Vy[i + sx * (J - L0)] = Vy[is + sx * (J - L0)];
                 }
             }
         }
         __syncthreads();
-        if (it + 1 < niter) {
-            // hand the first and last own rows to the neighbours
-            double* xp = xb + (size_t)(it & 1) * nb * 2 * xrow + (size_t)b * 2 * xrow;
-            for (int e = tid; e < 2 * W; e += nt) {
-                const int side = e / W, i = e % W;
-                const int l = i + sx * ((side == 0 ? r0 : r1 - 1) - L0);
-                double* dst = xp + (size_t)side * xrow + i;
+        if (st == H - 1 && it + 1 < niter) {
+            // hand the first and last H own rows to the neighbours
+            double* xp = xb + (size_t)((it / H) & 1) * nb * 2 * H * xrow + (size_t)b * 2 * H * xrow;
+            for (int e = tid; e < 2 * H * W; e += nt) {
+                const int side = e / (H * W), h = (e / W) % H, i = e % W;
+                const int l = i + sx * ((side == 0 ? r0 + h : r1 - H + h) - L0);
+                double* dst = xp + ((size_t)side * H + h) * xrow + i;
 #pragma unroll
                 for (int q = 0; q < kLdsState; ++q) dst[(size_t)q * sx] = smem[q * slot + l];
             }
             __syncthreads();
             if (tid == 0) {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                __hip_atomic_store(flags + b, gen0 + (unsigned long long)it + 1ull, __ATOMIC_RELAXED,
+                __hip_atomic_store(flags + b, gen0 + (unsigned long long)(it / H) + 1ull, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -889,13 +897,16 @@ static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepSc
 
 // Fields held in LDS per row by k2_lds, and its LDS bytes for R own rows.
 int lds2_fields(bool ve, bool bf) { return 9 + (bf ? 1 : 0) + (ve ? 3 : 0); }
-size_t lds2_bytes(int sx, int R, bool ve, bool bf) { return (size_t)lds2_fields(ve, bf) * (R + 2) * sx * sizeof(double); }
+size_t lds2_bytes(int sx, int R, int H, bool ve, bool bf) {
+    return (size_t)lds2_fields(ve, bf) * (R + 2 * H) * sx * sizeof(double);
+}
 
 template <bool VE, bool MUL, bool BF, bool DP>
-static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, int R, int niter,
-                         double* xb, unsigned long long* flags, unsigned long long gen0, int* err, int threads) {
-    auto kern = k2_lds<VE, MUL, BF, DP>;
-    const size_t bytes = lds2_bytes(f0.sx, R, VE, BF);
+static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, int R, int H,
+                         int niter, double* xb, unsigned long long* flags, unsigned long long gen0, int* err,
+                         int threads) {
+    auto kern = H == 2 ? k2_lds<VE, MUL, BF, DP, 2> : k2_lds<VE, MUL, BF, DP, 1>;
+    const size_t bytes = lds2_bytes(f0.sx, R, H, VE, BF);
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     const int nb = (f0.ny + 1 + R - 1) / R;
@@ -908,9 +919,9 @@ static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, cons
 }
 
 hipError_t launch2_iter_lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, bool ve,
-                            bool mul, bool bf, bool dp, int R, int niter, double* xb, unsigned long long* flags,
-                            unsigned long long gen0, int* err, int threads) {
-#define PTS_LDS_GO(V, M, B, D) return go2lds<V, M, B, D>(s, f0, f1, c, R, niter, xb, flags, gen0, err, threads)
+                            bool mul, bool bf, bool dp, int R, int H, int niter, double* xb,
+                            unsigned long long* flags, unsigned long long gen0, int* err, int threads) {
+#define PTS_LDS_GO(V, M, B, D) return go2lds<V, M, B, D>(s, f0, f1, c, R, H, niter, xb, flags, gen0, err, threads)
     if (ve) {
         if (mul) { if (bf) { if (dp) PTS_LDS_GO(true, true, true, true); PTS_LDS_GO(true, true, true, false); }
                    if (dp) PTS_LDS_GO(true, true, false, true); PTS_LDS_GO(true, true, false, false); }

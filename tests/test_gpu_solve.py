@@ -184,15 +184,18 @@ def test_cooperative_chunks_equal_single_launches(preset, kw, nout, rows, monkey
     ("Stokes2D_ve", dict(nx=6, ny=5, rad2=9.0, rhog_in=-1.0), 3, {}),
 ])
 @pytest.mark.parametrize("rows", ["0", "2", "7", "40"])
-def test_state_in_lds_chunks_equal_single_launches(preset, kw, nout, env, rows, monkeypatch):
+@pytest.mark.parametrize("halo", ["1", "2"])
+def test_state_in_lds_chunks_equal_single_launches(preset, kw, nout, env, rows, halo, monkeypatch):
     # small 2D grids: each chunk between checks runs with the state in LDS,
-    # blocks of `rows` rows (0: as many as fit; fewer when row ny-1 would
-    # start a block) exchanging their edge rows every iteration; the result
+    # blocks of `rows` rows (0: the default; fewer when a block's updated
+    # rows would start at row ny-1) exchanging their `halo` edge rows every
+    # `halo` iterations (computing the halo rows along); the result
     # equals one launch per iteration bit for bit, across odd and even chunk
     # lengths and repeated launches, over enough iterations for the flow to
     # reach the walls (3 * nout + 1 after 69)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
+    monkeypatch.setenv("PTS_2D_LDS_HALO", halo)
     monkeypatch.setenv("PTS_2D_LDS_ROWS", rows)
     monkeypatch.setenv("PTS_2D_LDS", "1")
     a = StokesModel(preset, nout=nout, **kw).setup()
