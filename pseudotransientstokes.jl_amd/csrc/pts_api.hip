@@ -64,8 +64,9 @@ struct pts_ctx {
     int* h_bar_err = nullptr;           // pinned copy of the error flag
     int planes_per_block = -160; // 3D single-pass: z planes per block (<0: fewest block rounds, capped)
     int dc_planes_per_block = -24; // the same for the 3D FP32 correction sweep
-    bool dc_dp = true;           // 3D FP32 sweep: Gdτ, dτ_Rho from Musτ when they are derived from it
-    bool dc_map_xcd = true;      // 3D FP32 sweep: each XCD sweeps its eighth of the tiles chunk by chunk
+    // 3D FP32 sweep (-1: when its arrays exceed the 256 MB Infinity Cache, 0 off, 1 on):
+    int dc_dp = -1;              // Gdτ, dτ_Rho from Musτ when they are derived from it
+    int dc_map_xcd = -1;         // each XCD sweeps its eighth of the tiles chunk by chunk
     int tile_rows_3d = 12;      // 3D single-pass: y rows per block (8, 12 or 16)
     bool two_kernel_3d = false; // 3D: in-place kernel pair instead of the single pass
     int variant_3d = 1;         // 3D single-pass kernel: 0 k3_iter, 1 k3_iter2, 2 k3_iter2 with prefetch
@@ -382,8 +383,8 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_2D_MULTI_ROWS")) c->multi_rows_2d = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("PTS_2D_MULTI_MAX_CELLS")) c->multi_max_cells = std::atoll(e);
     if (const char* e = std::getenv("PTS_DC_PLANES_PER_BLOCK")) c->dc_planes_per_block = std::atoi(e);
-    if (const char* e = std::getenv("PTS_DC_DP")) c->dc_dp = std::atoi(e) != 0;
-    if (const char* e = std::getenv("PTS_DC_MAP_XCD")) c->dc_map_xcd = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_DC_DP")) c->dc_dp = std::max(-1, std::min(1, std::atoi(e)));
+    if (const char* e = std::getenv("PTS_DC_MAP_XCD")) c->dc_map_xcd = std::max(-1, std::min(1, std::atoi(e)));
     if (const char* e = std::getenv("PTS_PLANES_PER_BLOCK")) c->planes_per_block = std::atoi(e);
     if (const char* e = std::getenv("PTS_3D_TWO_KERNEL")) c->two_kernel_3d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_3D_TILE_ROWS")) c->tile_rows_3d = std::atoi(e);
@@ -1301,8 +1302,8 @@ int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
     else if (k == "PTS_2D_LDS_HALO") c->lds_halo_2d = v == 1 ? 1 : 2;
     else if (k == "PTS_2D_MULTI_ROWS") c->multi_rows_2d = std::max(1, v);
     else if (k == "PTS_DC_PLANES_PER_BLOCK") c->dc_planes_per_block = v;
-    else if (k == "PTS_DC_DP") c->dc_dp = v != 0;
-    else if (k == "PTS_DC_MAP_XCD") c->dc_map_xcd = v != 0;
+    else if (k == "PTS_DC_DP") c->dc_dp = std::max(-1, std::min(1, v));
+    else if (k == "PTS_DC_MAP_XCD") c->dc_map_xcd = std::max(-1, std::min(1, v));
     else return fail(c, PTS_ERR_ARG, "unknown knob " + k);
     // captured graphs hold the old launch configuration
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
@@ -1430,7 +1431,14 @@ int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pt
     }
     // the 3D sweep derives Gdτ and dτ_Rho from Musτ when the context's are
     // derived from it (compute_iter_params!, no write since)
-    const bool dcdp = c->nd == 3 && c->dc_dp && use_dp(c);
+    // Where the sweep's 16 FP32 read arrays fit the 256 MB Infinity Cache it is
+    // compute-bound: the flat tile order and stored Gdτ, dτ_Rho are faster
+    // there (127^3: 0.066 against 0.078 ms per sweep); beyond, the XCD-wise
+    // order and Gdτ, dτ_Rho from Musτ (191^3: 0.258 against 0.269 ms; 511^3:
+    // 4.03 against 4.20 ms).
+    const bool big = c->nd == 3 && (double)c->nalloc * sizeof(float) * 16 > 256.0 * (1 << 20);
+    const bool dcdp = c->nd == 3 && (c->dc_dp < 0 ? big : c->dc_dp != 0) && use_dp(c);
+    const bool dcmap = c->dc_map_xcd < 0 ? big : c->dc_map_xcd != 0;
     const std::vector<int> consts = dcdp ? std::vector<int>{PTS_MUS, PTS_MUSTAU}
                                          : std::vector<int>{PTS_MUS, PTS_GDTAU, PTS_DTAU_RHO};
     for (int f : consts) {
@@ -1507,7 +1515,7 @@ int pts_dc_solve(pts_ctx* c, const pts_pt_params* p, const pts_dc_params* dp, pt
             const int64_t n = std::min<int64_t>(dp->nout_inner, dp->iter_max_inner - it);
             for (int64_t q = 0; q < n; ++q) {
                 HIPCHK(c, launch_dc_iter(c->stream, fields32_of(c, c->cur32), fields32_of(c, c->alt32), s32, c->nd,
-                                         c->dc_planes_per_block, dcdp, c->dc_map_xcd));
+                                         c->dc_planes_per_block, dcdp, dcmap));
                 for (int m = 0; m < ns; ++m) std::swap(c->cur32[st[m]], c->alt32[st[m]]);
             }
             it += n;
