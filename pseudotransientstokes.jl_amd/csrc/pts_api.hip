@@ -54,7 +54,7 @@ struct pts_ctx {
     bool lds_2d = true;                 // 2D small grids: state in LDS for a check interval (k2_lds)
     int lds_rows_2d = 0;                // rows per block of that launch (0: the default choice)
     int lds_threads_2d = 1024;          // threads per block of that launch
-    int lds_halo_2d = 2;                // halo rows (1 or 2): the blocks exchange rows every that many iterations
+    int lds_halo_2d = 0;                // halo rows (1 to 4; 0: the default choice): rows exchanged every that many iterations
                                         // (127^2: 5.4 us per iteration with 2, 6.2 with 1)
     double* d_lds_xb = nullptr;         // its exchange rows
     unsigned long long* d_lds_flags = nullptr;
@@ -379,7 +379,7 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_2D_LDS")) c->lds_2d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_2D_LDS_ROWS")) c->lds_rows_2d = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("PTS_2D_LDS_THREADS")) c->lds_threads_2d = std::min(1024, std::max(64, std::atoi(e) / 64 * 64));
-    if (const char* e = std::getenv("PTS_2D_LDS_HALO")) c->lds_halo_2d = std::atoi(e) == 1 ? 1 : 2;
+    if (const char* e = std::getenv("PTS_2D_LDS_HALO")) c->lds_halo_2d = std::min(4, std::max(0, std::atoi(e)));
     if (const char* e = std::getenv("PTS_2D_MULTI_ROWS")) c->multi_rows_2d = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("PTS_2D_MULTI_MAX_CELLS")) c->multi_max_cells = std::atoll(e);
     if (const char* e = std::getenv("PTS_DC_PLANES_PER_BLOCK")) c->dc_planes_per_block = std::atoi(e);
@@ -966,12 +966,14 @@ static int run_iters_multi(pts_ctx* c, const StepScalars& s0, int64_t n) {
     return 1;
 }
 
+constexpr int kLdsMaxHalo = 4;  // deepest halo (PTS_2D_LDS_HALO); sizes the exchange buffer
+
 // 2D small grids: n iterations with the state in LDS in one cooperative
-// launch (k2_lds; 1: done; 0: not applicable).  R rows per block: 4 (or the
-// nearest count that works), at least 2, at most what the LDS holds
-// (160 KiB per CU) with H halo rows on each side, such that the rows a block
-// updates never start at row ny-1 (bc_y!(Vx) copies row ny-2 into it), and
-// few enough blocks to be resident at once.
+// launch (k2_lds; 1: done; 0: not applicable).  Blocks of R rows, the last
+// one owning the rows left (R .. 2R-1, or more when it takes over the rows
+// of the block before, below): R as asked (or the next fewer that works),
+// at least H, within what the LDS holds (160 KiB per CU) with H halo rows
+// on each side, and few enough blocks to be resident at once.
 static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     if (c->nd != 2 || !c->lds_2d || !single_pass(c) || n < 2 || n > (1 << 30)) return 0;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -983,28 +985,39 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     const bool dp = use_dp(c);
     const size_t lds_max = 160 * 1024;
     const size_t per_row = (size_t)lds2_fields(ve, bf) * c->sx * sizeof(double);  // a row of every held field
-    const int H = c->lds_halo_2d;
-    const int fit = (int)(lds_max / per_row) - 2 * H;
     // the rows a block updates in the steps of an exchange period start at
-    // r0 - (H-1) .. r0: none may be row ny-1 (bc_y!(Vx) copies row ny-2 into it)
-    auto works = [&](int r) {
-        for (int g = 0; g < H; ++g)
-            if ((c->ny - 1 + g) % r == 0) return false;
-        return true;
+    // r0 - (H-1) .. r0: none may be row ny-1 (bc_y!(Vx) copies row ny-2 into
+    // it).  Only the last block's start can be (it is at most ny+1-R): then
+    // the last block takes over the rows of the one before.
+    auto blocks = [&](int r, int h) {
+        int b = std::max(1, (c->ny + 1) / r);
+        while (b > 1 && (b - 1) * r >= c->ny - 1 && (b - 1) * r <= c->ny + h - 2) --b;
+        return b;
     };
-    int R = 0;
-    if (c->lds_rows_2d > 0) {  // as asked, or the next fewer that works
-        for (int r = std::min(fit, c->lds_rows_2d); r >= 2 && !R; --r)
-            if (works(r)) R = r;
-    } else {  // short blocks: more of them in parallel (127^2: 4 rows 5.4 us, 8 rows 6.8 us per iteration)
-        for (int r : {4, 3, 5, 6, 7, 8, 2})
-            if (r <= fit && works(r)) {
-                R = r;
+    auto works = [&](int r, int h) {
+        const int fit = (int)(lds_max / per_row) - 2 * h;  // rows a block can own
+        return r >= 2 && r >= h && r <= c->ny + 1 && c->ny + 1 - (blocks(r, h) - 1) * r <= fit;
+    };
+    int R = 0, H = 0;
+    if (c->lds_rows_2d > 0 || c->lds_halo_2d > 0) {  // as asked, or the next fewer rows that work
+        H = c->lds_halo_2d > 0 ? c->lds_halo_2d : 2;
+        for (int r = c->lds_rows_2d > 0 ? c->lds_rows_2d : 4; r >= 2 && !R; --r)
+            if (works(r, H)) R = r;
+    } else {
+        // short blocks, exchanged every few iterations: many blocks in parallel
+        // against the latency of an exchange through memory (127^2: 3 rows
+        // every 3 iterations 4.8 us per iteration, 4 every 2 5.4, 3 every 1
+        // 6.3, 6 every 4 6.1; 255^2: 3 every 2 8.7 us)
+        const int pairs[][2] = {{3, 3}, {3, 2}, {4, 2}, {2, 2}, {5, 2}, {6, 2}, {7, 2}, {8, 2}};
+        for (const auto& p : pairs)
+            if (works(p[0], p[1])) {
+                R = p[0];
+                H = p[1];
                 break;
             }
     }
     if (R < 2) return 0;
-    const int nb = (c->ny + 1 + R - 1) / R;
+    const int nb = blocks(R, H);
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1029,13 +1042,13 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
         c->d_lds_xb = nullptr;
         c->d_lds_flags = nullptr;
         c->lds_blocks = 0;
-        HIPCHK(c, hipMalloc((void**)&c->d_lds_xb, (size_t)2 * nb * 2 * 2 * 6 * c->sx * sizeof(double)));
+        HIPCHK(c, hipMalloc((void**)&c->d_lds_xb, (size_t)2 * nb * 2 * kLdsMaxHalo * 6 * c->sx * sizeof(double)));
         HIPCHK(c, hipMalloc((void**)&c->d_lds_flags, (size_t)nb * sizeof(unsigned long long)));
         HIPCHK(c, hipMemsetAsync(c->d_lds_flags, 0, (size_t)nb * sizeof(unsigned long long), c->stream));
         c->lds_blocks = nb;
         c->lds_gen = 0;
     }
-    hipError_t e = launch2_iter_lds(c->stream, f0, f1, with_dp(c, s0), ve, mul, bf, dp, R, H, (int)n, c->d_lds_xb,
+    hipError_t e = launch2_iter_lds(c->stream, f0, f1, with_dp(c, s0), ve, mul, bf, dp, R, nb, H, (int)n, c->d_lds_xb,
                                     c->d_lds_flags, c->lds_gen, (int*)(c->d_bar + 2), c->lds_threads_2d);
     if (e == hipErrorNotSupported || e == hipErrorCooperativeLaunchTooLarge || e == hipErrorInvalidValue) {
         (void)hipGetLastError();
@@ -1299,7 +1312,7 @@ int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
     else if (k == "PTS_2D_LDS") c->lds_2d = v != 0;
     else if (k == "PTS_2D_LDS_ROWS") c->lds_rows_2d = std::max(0, v);
     else if (k == "PTS_2D_LDS_THREADS") c->lds_threads_2d = std::min(1024, std::max(64, v / 64 * 64));
-    else if (k == "PTS_2D_LDS_HALO") c->lds_halo_2d = v == 1 ? 1 : 2;
+    else if (k == "PTS_2D_LDS_HALO") c->lds_halo_2d = (int)std::min<int64_t>(4, std::max<int64_t>(0, v));
     else if (k == "PTS_2D_MULTI_ROWS") c->multi_rows_2d = std::max(1, v);
     else if (k == "PTS_DC_PLANES_PER_BLOCK") c->dc_planes_per_block = v;
     else if (k == "PTS_DC_DP") c->dc_dp = std::max(-1, std::min(1, v));

@@ -192,15 +192,16 @@ hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, cons
 // pts_pt_iterate's ping-pong does.  hipErrorNotSupported when the tiles of
 // `rows` rows do not fit one round of resident blocks.
 // Small 2D grids with the state in LDS (k2_lds): niter iterations in one
-// cooperative launch of ceil((ny+1)/R) blocks of R rows, exchanging H rows
-// (1 or 2) every H iterations; xb holds 2 * blocks * 2 * H * 6 * sx
-// doubles, flags one counter per block (gen0: their value when the launch
-// starts; they end at gen0 + (niter - 1) / H); threads per block: a
-// multiple of 64 up to 1024.
+// cooperative launch of nb blocks of R rows (the last one: the rows left,
+// at least R), exchanging H rows (1 to 4, at most R) every H iterations; xb
+// holds 2 * nb * 2 * H * 6 * sx doubles, flags one counter per block (gen0:
+// their value when the launch starts; they end at gen0 + (niter - 1) / H);
+// threads per block: a multiple of 64 up to 1024.  lds2_bytes: the LDS of a
+// block that owns `rows` rows.
 int lds2_fields(bool ve, bool bf);
-size_t lds2_bytes(int sx, int R, int H, bool ve, bool bf);
+size_t lds2_bytes(int sx, int rows, int H, bool ve, bool bf);
 hipError_t launch2_iter_lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, bool ve,
-                            bool mul, bool bf, bool dp, int R, int H, int niter, double* xb,
+                            bool mul, bool bf, bool dp, int R, int nb, int H, int niter, double* xb,
                             unsigned long long* flags, unsigned long long gen0, int* err, int threads);
 hipError_t launch2_iter_multi(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
                               bool mul, bool bf, int rows, int niter, unsigned* bar, int* err);

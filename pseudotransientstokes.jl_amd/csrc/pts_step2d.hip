@@ -557,8 +557,9 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
 
 // ---------------------------------------------------------------------------
 // Small 2D grids: the whole state in LDS for a whole check interval.  Block b
-// holds rows [r0, r1) = [b·R, b·R + R) of every field, plus H halo rows on
-// each side, and runs `niter` iterations on them: the updates of
+// holds rows [r0, r1) = [b·R, b·R + R) of every field (the last block: from
+// b·R up to row ny, at least R rows), plus H halo rows on each side, and runs
+// `niter` iterations on them: the updates of
 // compute_P!/compute_τ! (cells and vertices, in place) and of compute_V!
 // (faces, in place, with the free-slip copies of bc_y!(Vx), bc_x!(Vy)), each
 // a phase between barriers.  Every H iterations a block hands its first and
@@ -589,8 +590,8 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
     extern __shared__ double smem[];
     const int nx = f0.nx, ny = f0.ny, sx = f0.sx;
     const int nb = gridDim.x, b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-    const int r0 = b * R, r1 = min(r0 + R, ny + 1);  // own rows (Vy has faces up to row ny)
-    const int L0 = r0 - H, NL = R + 2 * H;            // LDS rows L0 .. L0 + NL - 1
+    const int r0 = b * R, r1 = b == nb - 1 ? ny + 1 : r0 + R;  // own rows (Vy has faces up to row ny)
+    const int L0 = r0 - H, NL = ny + 1 - (nb - 1) * R + 2 * H;  // LDS rows L0 .. L0 + NL - 1 (the last block's count)
     const int lb = L0 * sx;                           // global index of local index 0
     const size_t slot = (size_t)NL * sx;
     double* const Pt = smem;
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
     lf.p[PTS_TXY] = txy;
     lf.p[PTS_RHOG] = rg;
     const int W = nx + 1;  // columns of the widest field
-    const int ja = max(L0, 0), jz = min(L0 + NL - 1, ny);  // rows held
+    const int ja = max(L0, 0), jz = min(r1 - 1 + H, ny);  // rows held
 
     // load the tile (rows outside the grid are never read)
     for (int e = tid; e < (jz - ja + 1) * W; e += nt) {
@@ -897,19 +898,22 @@ static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepSc
 
 // Fields held in LDS per row by k2_lds, and its LDS bytes for R own rows.
 int lds2_fields(bool ve, bool bf) { return 9 + (bf ? 1 : 0) + (ve ? 3 : 0); }
-size_t lds2_bytes(int sx, int R, int H, bool ve, bool bf) {
-    return (size_t)lds2_fields(ve, bf) * (R + 2 * H) * sx * sizeof(double);
+size_t lds2_bytes(int sx, int rows, int H, bool ve, bool bf) {
+    return (size_t)lds2_fields(ve, bf) * (rows + 2 * H) * sx * sizeof(double);
 }
 
 template <bool VE, bool MUL, bool BF, bool DP>
-static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, int R, int H,
-                         int niter, double* xb, unsigned long long* flags, unsigned long long gen0, int* err,
+static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, int R, int nb,
+                         int H, int niter, double* xb, unsigned long long* flags, unsigned long long gen0, int* err,
                          int threads) {
-    auto kern = H == 2 ? k2_lds<VE, MUL, BF, DP, 2> : k2_lds<VE, MUL, BF, DP, 1>;
-    const size_t bytes = lds2_bytes(f0.sx, R, H, VE, BF);
+    auto kern = H == 4   ? k2_lds<VE, MUL, BF, DP, 4>
+                : H == 3 ? k2_lds<VE, MUL, BF, DP, 3>
+                : H == 2 ? k2_lds<VE, MUL, BF, DP, 2>
+                         : k2_lds<VE, MUL, BF, DP, 1>;
+    if (nb < 1 || R < H || f0.ny + 1 - (nb - 1) * R < R) return hipErrorInvalidValue;
+    const size_t bytes = lds2_bytes(f0.sx, f0.ny + 1 - (nb - 1) * R, H, VE, BF);
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
-    const int nb = (f0.ny + 1 + R - 1) / R;
     Fields a = f0, bb = f1;
     StepScalars cc = c;
     int rr = R, ni = niter;
@@ -919,9 +923,10 @@ static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, cons
 }
 
 hipError_t launch2_iter_lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, bool ve,
-                            bool mul, bool bf, bool dp, int R, int H, int niter, double* xb,
+                            bool mul, bool bf, bool dp, int R, int nb, int H, int niter, double* xb,
                             unsigned long long* flags, unsigned long long gen0, int* err, int threads) {
-#define PTS_LDS_GO(V, M, B, D) return go2lds<V, M, B, D>(s, f0, f1, c, R, H, niter, xb, flags, gen0, err, threads)
+#define PTS_LDS_GO(V, M, B, D) \
+    return go2lds<V, M, B, D>(s, f0, f1, c, R, nb, H, niter, xb, flags, gen0, err, threads)
     if (ve) {
         if (mul) { if (bf) { if (dp) PTS_LDS_GO(true, true, true, true); PTS_LDS_GO(true, true, true, false); }
                    if (dp) PTS_LDS_GO(true, true, false, true); PTS_LDS_GO(true, true, false, false); }

@@ -183,13 +183,15 @@ def test_cooperative_chunks_equal_single_launches(preset, kw, nout, rows, monkey
     ("Stokes2D_ve", dict(n=63, dt=0.1), 200, {}),
     ("Stokes2D_ve", dict(nx=6, ny=5, rad2=9.0, rhog_in=-1.0), 3, {}),
 ])
-@pytest.mark.parametrize("rows", ["0", "2", "7", "40"])
-@pytest.mark.parametrize("halo", ["1", "2"])
+@pytest.mark.parametrize("rows", ["0", "2", "3", "7", "40"])
+@pytest.mark.parametrize("halo", ["0", "1", "2", "3", "4"])
 def test_state_in_lds_chunks_equal_single_launches(preset, kw, nout, env, rows, halo, monkeypatch):
     # small 2D grids: each chunk between checks runs with the state in LDS,
-    # blocks of `rows` rows (0: the default; fewer when a block's updated
-    # rows would start at row ny-1) exchanging their `halo` edge rows every
-    # `halo` iterations (computing the halo rows along); the result
+    # blocks of `rows` rows (at least `halo`; the last block owns the rows
+    # left, and those of the block before when a block's updated rows would
+    # start at row ny-1) exchanging their `halo` edge rows every `halo`
+    # iterations (computing the halo rows along; 0 and 0: the default
+    # choice of both); the result
     # equals one launch per iteration bit for bit, across odd and even chunk
     # lengths and repeated launches, over enough iterations for the flow to
     # reach the walls (3 * nout + 1 after 69)
