@@ -60,6 +60,7 @@ struct pts_ctx {
     unsigned long long* d_lds_flags = nullptr;
     int lds_blocks = 0;                 // blocks the buffers are sized for
     unsigned long long lds_gen = 0;     // the counters' value between launches
+    int lds_nb = 0;                     // blocks of the last launch (whose counters all hold lds_gen)
     unsigned* d_bar = nullptr;          // barrier count, generation, error flag
     int* h_bar_err = nullptr;           // pinned copy of the error flag
     int planes_per_block = -160; // 3D single-pass: z planes per block (<0: fewest block rounds, capped)
@@ -1047,6 +1048,12 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
         HIPCHK(c, hipMemsetAsync(c->d_lds_flags, 0, (size_t)nb * sizeof(unsigned long long), c->stream));
         c->lds_blocks = nb;
         c->lds_gen = 0;
+        c->lds_nb = nb;
+    }
+    if (nb != c->lds_nb) {  // another block count (a knob changed): the counters start again at zero
+        HIPCHK(c, hipMemsetAsync(c->d_lds_flags, 0, (size_t)c->lds_blocks * sizeof(unsigned long long), c->stream));
+        c->lds_gen = 0;
+        c->lds_nb = nb;
     }
     hipError_t e = launch2_iter_lds(c->stream, f0, f1, with_dp(c, s0), ve, mul, bf, dp, R, nb, H, (int)n, c->d_lds_xb,
                                     c->d_lds_flags, c->lds_gen, (int*)(c->d_bar + 2), c->lds_threads_2d);
@@ -1055,7 +1062,7 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
         return 0;
     }
     if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("cooperative launch: ") + hipGetErrorString(e));
-    // blocks that no block waits on (the flags of a grid of one block) stay put
+    // every block publishes (n - 1) / H times
     c->lds_gen += (unsigned long long)((n - 1) / H);
     e = hipMemcpyAsync(c->h_bar_err, c->d_bar + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("exchange flag: ") + hipGetErrorString(e));

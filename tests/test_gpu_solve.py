@@ -216,3 +216,25 @@ def test_state_in_lds_chunks_equal_single_launches(preset, kw, nout, env, rows, 
             assert np.array_equal(a.get(f), b.get(f)), (f, it_max)
     a.close()
     b.close()
+
+
+def test_state_in_lds_block_count_changes_between_launches(monkeypatch):
+    # the exchange counters of one context stay consistent when the block
+    # count changes between launches (knobs set on the live context)
+    kw = dict(n=127, rhog_in=-1.0, ebg=0.0, nout=10)
+    monkeypatch.setenv("PTS_2D_LDS", "1")
+    a = StokesModel("Stokes2D", **kw).setup()
+    monkeypatch.setenv("PTS_2D_LDS", "0")
+    monkeypatch.setenv("PTS_2D_MULTI", "0")
+    b = StokesModel("Stokes2D", **kw).setup()
+    params = a.pt_params()
+    for rows, halo in ((3, 3), (7, 2), (3, 3), (0, 0), (2, 1)):
+        a.ctx.set_knob("PTS_2D_LDS_ROWS", rows)
+        a.ctx.set_knob("PTS_2D_LDS_HALO", halo)
+        ra = a.ctx.pt_solve(params, 1e-300, 10, 39, 1, use_graph=True)
+        rb = b.ctx.pt_solve(params, 1e-300, 10, 39, 1, use_graph=True)
+        assert ra["iters"] == rb["iters"] and ra["err"] == rb["err"], (rows, halo, ra, rb)
+        for f in ("Pt", "Vx", "Vy", "txx", "tyy", "txy"):
+            assert np.array_equal(a.get(f), b.get(f)), (f, rows, halo)
+    a.close()
+    b.close()
