@@ -1007,8 +1007,8 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     } else {
         // short blocks, exchanged every few iterations: many blocks in parallel
         // against the latency of an exchange through memory (127^2: 3 rows
-        // every 3 iterations 4.8 us per iteration, 4 every 2 5.4, 3 every 1
-        // 6.3, 6 every 4 6.1; 255^2: 3 every 2 8.7 us)
+        // every 3 iterations 4.6 us per iteration, 4 every 3 4.9, 3 every 2
+        // 4.9, 4 every 2 5.1, 3 every 1 6.3; 255^2: 3 every 2 7.9 us)
         const int pairs[][2] = {{3, 3}, {3, 2}, {4, 2}, {2, 2}, {5, 2}, {6, 2}, {7, 2}, {8, 2}};
         for (const auto& p : pairs)
             if (works(p[0], p[1])) {

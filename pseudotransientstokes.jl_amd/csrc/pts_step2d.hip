@@ -647,7 +647,7 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
         // shrink by one per step on each side (g rows beyond the own ones)
         const int st = it % H, g = H - 1 - st;
         const int ca = max(r0 - H + st, 0), cz = min(r1 - 1 + g, ny - 1);  // cell rows of the P, τ phase
-        const int va = ca, vz = min(r1 - 1 + g, ny - 2);                   // vertex rows
+        const int vz = min(r1 - 1 + g, ny - 2);                            // vertex rows ca .. vz
         const int xa = max(r0 - g, 1), xz = min(r1 - 1 + g, ny - 2);       // Vx face rows updated
         const int ya = max(r0 - g, 1), yz = min(r1 - 1 + g, ny - 1);       // Vy face rows updated
         const int ba = max(r0 - g, 0), bz = min(r1 - 1 + g, ny);           // rows of the free-slip copies
@@ -692,11 +692,14 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
             }
         }
         // compute_P!, compute_τ! (Stokes2D.jl:33-44, Stokes2D_ve.jl:42-55,
-        // Stokes2D_ve_perf.jl:47-53): cells and vertices, in place
-        const int ncell = max(cz - ca + 1, 0) * nx, nvert = max(vz - va + 1, 0) * (nx - 1);
-        for (int e = tid; e < ncell + nvert; e += nt) {
-            if (e < ncell) {
-                const int J = ca + e / nx, i = e % nx, o = i + sx * (J - L0);
+        // Stokes2D_ve_perf.jl:47-53): cells and vertices, in place; a thread
+        // updates cell (i, J) and vertex (i, J) together (the vertex rows
+        // ca..vz lie within the cell rows ca..cz), two
+        // independent chains in one pass over the rows
+        const int ncell = max(cz - ca + 1, 0) * nx;
+        for (int e = tid; e < ncell; e += nt) {
+            const int J = ca + e / nx, i = e % nx, o = i + sx * (J - L0);
+            {
                 const double g = Gd[o], m = Mu[o];
                 if constexpr (MUL) {
                     const double ex = c._dx * (Vx[o + 1] - Vx[o]), ey = c._dy * (Vy[o + sx] - Vy[o]);
@@ -716,8 +719,8 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
                         tyy[o] = tau_visc2d_div(tyy[o], g, m, ey, c.dy);
                     }
                 }
-            } else {
-                const int e2 = e - ncell, J = va + e2 / (nx - 1), i = e2 % (nx - 1), o = i + sx * (J - L0);
+            }
+            if (J <= vz && i < nx - 1) {
                 const double ag = (Gd[o] + Gd[o + 1] + Gd[o + sx] + Gd[o + sx + 1]) * 0.25;
                 const double am = (Mu[o] + Mu[o + 1] + Mu[o + sx] + Mu[o + sx + 1]) * 0.25;
                 const double ev = 0.5 * (dfd<MUL>(Vx[o + sx + 1] - Vx[o + 1], c.dy, c._dy) +
@@ -733,24 +736,28 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
         // stored with it (this block updates the row it copies: R >= 2, and
         // the updated rows never start at ny-1), and the copies of the
         // Dirichlet corners (columns 0, nx of Vx; rows 0, ny of Vy) follow
-        const int nvx = max(xz - xa + 1, 0) * (nx - 1), nvy = max(yz - ya + 1, 0) * (nx - 2);
-        for (int e = tid; e < nvx + nvy + 8; e += nt) {
-            if (e < nvx) {
-                const int J = xa + e / (nx - 1), i = 1 + e % (nx - 1), o = (i - 1) + sx * (J - 1 - L0);
-                const double Rr = res2_x<MUL>(lf, c, o);
-                const double v = Vx[o + sx + 1] + Rr * ((Rh[o + sx] + Rh[o + sx + 1]) * 0.5);
-                Vx[o + sx + 1] = v;
-                if (J == 1) Vx[o + 1] = v;                   // Vx[:, 0] = Vx[:, 1]
-                if (J == ny - 2) Vx[o + 2 * sx + 1] = v;     // Vx[:, ny-1] = Vx[:, ny-2]
-            } else if (e < nvx + nvy) {
-                const int e2 = e - nvx, J = ya + e2 / (nx - 2), i = 1 + e2 % (nx - 2), o = (i - 1) + sx * (J - 1 - L0);
-                const double Rr = res2_y<MUL>(lf, c, o, BF);
-                const double v = Vy[o + sx + 1] + Rr * ((Rh[o + 1] + Rh[o + sx + 1]) * 0.5);
-                Vy[o + sx + 1] = v;
-                if (i == 1) Vy[o + sx] = v;                  // Vy[0, :] = Vy[1, :]
-                if (i == nx - 2) Vy[o + sx + 2] = v;         // Vy[nx-1, :] = Vy[nx-2, :]
+        // (a thread updates Vx and Vy of face (i, J) together: the Vx rows
+        // xa..xz lie within the Vy rows ya..yz, xa = ya, xz <= yz)
+        const int nface = max(yz - ya + 1, 0) * (nx - 1);
+        for (int e = tid; e < nface + 8; e += nt) {
+            if (e < nface) {
+                const int J = ya + e / (nx - 1), i = 1 + e % (nx - 1), o = (i - 1) + sx * (J - 1 - L0);
+                if (J <= xz) {
+                    const double Rr = res2_x<MUL>(lf, c, o);
+                    const double v = Vx[o + sx + 1] + Rr * ((Rh[o + sx] + Rh[o + sx + 1]) * 0.5);
+                    Vx[o + sx + 1] = v;
+                    if (J == 1) Vx[o + 1] = v;                   // Vx[:, 0] = Vx[:, 1]
+                    if (J == ny - 2) Vx[o + 2 * sx + 1] = v;     // Vx[:, ny-1] = Vx[:, ny-2]
+                }
+                if (i < nx - 1) {
+                    const double Rr = res2_y<MUL>(lf, c, o, BF);
+                    const double v = Vy[o + sx + 1] + Rr * ((Rh[o + 1] + Rh[o + sx + 1]) * 0.5);
+                    Vy[o + sx + 1] = v;
+                    if (i == 1) Vy[o + sx] = v;                  // Vy[0, :] = Vy[1, :]
+                    if (i == nx - 2) Vy[o + sx + 2] = v;         // Vy[nx-1, :] = Vy[nx-2, :]
+                }
             } else {
-                const int k = e - nvx - nvy;
+                const int k = e - nface;
                 if (k < 4) {  // Vx at the Dirichlet columns 0 and nx, rows 0 and ny-1
                     const int i = (k & 1) ? nx : 0, J = k < 2 ? 0 : ny - 1, Js = k < 2 ? 1 : ny - 2;
                     if (J >= ba && J <= bz) Vx[i + sx * (J - L0)] = Vx[i + sx * (Js - L0)];
