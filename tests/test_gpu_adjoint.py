@@ -98,6 +98,47 @@ def test_adjoint_step_is_transpose_of_oracle_iteration(nx, ny, ve):
     c.close()
 
 
+@pytest.mark.parametrize("ve", [0, 1])
+def test_adjoint_step_is_transpose_at_2047(ve):
+    # the inversion's full size: with zero Dirichlet values, no body force and
+    # τ_o = 0 one forward iteration is a linear map M of the state, and one
+    # adjoint step must be its transpose: <M x, y> = <x, Mᵀ y> for seeded
+    # random states x, y and random positive coefficients (the forward is the
+    # single-pass iteration, the adjoint the single-pass adjoint sweep)
+    n = 2047
+    rng = np.random.default_rng(2047 + ve)
+    G, dt = 0.7, 0.5
+    dx = dy = 10.0 / n
+    c = Context(2, n, n, ve=bool(ve), fd_mode="mul")
+    coef = {"Mus": np.exp(rng.normal(0.0, 1.0, (n, n))), "Gdtau": rng.uniform(0.2, 2.0, (n, n)),
+            "dtau_Rho": rng.uniform(0.01, 0.2, (n, n))}
+    for name, a in coef.items():
+        c.upload(name, a)
+    if ve:
+        for q in ("txx_o", "tyy_o", "txy_o"):
+            c.upload(q, np.zeros(c.shape(q)))
+    params = Context.pt_params(1.0, G, dt, dx, dy)
+    x = {q: rng.normal(size=c.shape(q)) * state_mask(q, c.shape(q)) for q in ADJ}
+    y = {q: rng.normal(size=c.shape(q)) * state_mask(q, c.shape(q)) for q in ADJ}
+    for q in ADJ:
+        c.upload(q, x[q])
+    c.pt_iterate(params, 1)
+    Mx = {q: c.download(q) for q in ADJ}
+    c.adjoint_set_source(None, None)
+    for q in ADJ:
+        c.adjoint_upload(q, y[q])
+    c.adjoint_iterate(params, 1)
+    MTy = {q: c.adjoint_download(q) for q in ADJ}
+    c.close()
+    lhs = sum(float(np.sum(Mx[q][state_mask(q, Mx[q].shape)] * y[q][state_mask(q, Mx[q].shape)])) for q in ADJ)
+    rhs = sum(float(np.sum(x[q] * MTy[q])) for q in ADJ)
+    # scale: the sum of the magnitudes of the terms (rounding of the dot products)
+    mag = sum(float(np.sum(np.abs(x[q] * MTy[q]))) for q in ADJ)
+    assert abs(lhs - rhs) <= 1e-13 * mag, (lhs, rhs, mag)
+    for q in ("Vx", "Vy"):
+        assert np.all(MTy[q][~state_mask(q, MTy[q].shape)] == 0.0)
+
+
 @pytest.mark.parametrize("nx,ny,ve", [(131, 70, 0), (61, 200, 1), (7, 5, 0), (300, 9, 1)])
 def test_fused_adjoint_sweep_equals_two_passes(nx, ny, ve, monkeypatch):
     """The single-pass adjoint sweep gives the bits of the two-pass form,
