@@ -58,7 +58,8 @@ struct pts_ctx {
                                         // (127^2: 5.4 us per iteration with 2, 6.2 with 1)
     double* d_lds_xb = nullptr;         // its exchange rows
     unsigned long long* d_lds_flags = nullptr;
-    int lds_blocks = 0;                 // blocks the buffers are sized for
+    int lds_blocks = 0;                 // blocks the counters are sized for
+    size_t lds_xb_bytes = 0;            // size of the exchange buffer
     unsigned long long lds_gen = 0;     // the counters' value between launches
     int lds_nb = 0;                     // blocks of the last launch (whose counters all hold lds_gen)
     unsigned* d_bar = nullptr;          // barrier count, generation, error flag
@@ -967,14 +968,13 @@ static int run_iters_multi(pts_ctx* c, const StepScalars& s0, int64_t n) {
     return 1;
 }
 
-constexpr int kLdsMaxHalo = 4;  // deepest halo (PTS_2D_LDS_HALO); sizes the exchange buffer
-
 // 2D small grids: n iterations with the state in LDS in one cooperative
 // launch (k2_lds; 1: done; 0: not applicable).  Blocks of R rows, the last
 // one owning the rows left (R .. 2R-1, or more when it takes over the rows
-// of the block before, below): R as asked (or the next fewer that works),
-// at least H, within what the LDS holds (160 KiB per CU) with H halo rows
-// on each side, and few enough blocks to be resident at once.
+// of blocks before it, below): R as asked (or the next fewer that works),
+// at least 2, within what the LDS holds (160 KiB per CU) with H halo rows
+// on each side (H may exceed R), and few enough blocks to be resident at
+// once.
 static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     if (c->nd != 2 || !c->lds_2d || !single_pass(c) || n < 2 || n > (1 << 30)) return 0;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -997,7 +997,7 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     };
     auto works = [&](int r, int h) {
         const int fit = (int)(lds_max / per_row) - 2 * h;  // rows a block can own
-        return r >= 2 && r >= h && r <= c->ny + 1 && c->ny + 1 - (blocks(r, h) - 1) * r <= fit;
+        return r >= 2 && r <= c->ny + 1 && c->ny + 1 - (blocks(r, h) - 1) * r <= fit;
     };
     int R = 0, H = 0;
     if (c->lds_rows_2d > 0 || c->lds_halo_2d > 0) {  // as asked, or the next fewer rows that work
@@ -1007,8 +1007,8 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     } else {
         // short blocks, exchanged every few iterations: many blocks in parallel
         // against the latency of an exchange through memory (127^2: 3 rows
-        // every 3 iterations 4.6 us per iteration, 4 every 3 4.9, 3 every 2
-        // 4.9, 4 every 2 5.1, 3 every 1 6.3; 255^2: 3 every 2 7.9 us)
+        // every 3 iterations 4.4 us per iteration, 2 every 3 or 4 4.5, 4
+        // every 4 4.8, 2 every 2 4.9; 255^2: 3 every 2 7.5 us)
         const int pairs[][2] = {{3, 3}, {3, 2}, {4, 2}, {2, 2}, {5, 2}, {6, 2}, {7, 2}, {8, 2}};
         for (const auto& p : pairs)
             if (works(p[0], p[1])) {
@@ -1037,13 +1037,20 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
         if (!f0.p[f]) return 0;
     for (int q = 0; q < ns; ++q)
         if (!f1.p[st[q]]) return 0;
-    if (nb > c->lds_blocks) {
+    // the exchange buffer: two parities of every block's rows (each block
+    // at the stride of the last block's row count)
+    const size_t xb_bytes = (size_t)2 * nb * (c->ny + 1 - (nb - 1) * R) * 6 * c->sx * sizeof(double);
+    if (xb_bytes > c->lds_xb_bytes) {
         if (c->d_lds_xb) (void)hipFree(c->d_lds_xb);
-        if (c->d_lds_flags) (void)hipFree(c->d_lds_flags);
         c->d_lds_xb = nullptr;
+        c->lds_xb_bytes = 0;
+        HIPCHK(c, hipMalloc((void**)&c->d_lds_xb, xb_bytes));
+        c->lds_xb_bytes = xb_bytes;
+    }
+    if (nb > c->lds_blocks) {
+        if (c->d_lds_flags) (void)hipFree(c->d_lds_flags);
         c->d_lds_flags = nullptr;
         c->lds_blocks = 0;
-        HIPCHK(c, hipMalloc((void**)&c->d_lds_xb, (size_t)2 * nb * 2 * kLdsMaxHalo * 6 * c->sx * sizeof(double)));
         HIPCHK(c, hipMalloc((void**)&c->d_lds_flags, (size_t)nb * sizeof(unsigned long long)));
         HIPCHK(c, hipMemsetAsync(c->d_lds_flags, 0, (size_t)nb * sizeof(unsigned long long), c->stream));
         c->lds_blocks = nb;

@@ -192,9 +192,10 @@ hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, cons
 // pts_pt_iterate's ping-pong does.  hipErrorNotSupported when the tiles of
 // `rows` rows do not fit one round of resident blocks.
 // Small 2D grids with the state in LDS (k2_lds): niter iterations in one
-// cooperative launch of nb blocks of R rows (the last one: the rows left,
-// at least R), exchanging H rows (1 to 4, at most R) every H iterations; xb
-// holds 2 * nb * 2 * H * 6 * sx doubles, flags one counter per block (gen0:
+// cooperative launch of nb blocks of R >= 2 rows (the last one: the rows
+// left, at least R), exchanging H rows (1 to 4) every H iterations; xb
+// holds 2 * nb * RL * 6 * sx doubles (RL = ny + 1 - (nb - 1) * R, the last
+// block's rows), flags one counter per block (gen0:
 // their value when the launch starts; they end at gen0 + (niter - 1) / H);
 // threads per block: a multiple of 64 up to 1024.  lds2_bytes: the LDS of a
 // block that owns `rows` rows.

@@ -562,10 +562,11 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
 // `niter` iterations on them: the updates of
 // compute_P!/compute_τ! (cells and vertices, in place) and of compute_V!
 // (faces, in place, with the free-slip copies of bc_y!(Vx), bc_x!(Vy)), each
-// a phase between barriers.  Every H iterations a block hands its first and
-// last H own rows to its neighbours through a small global buffer (two
-// parities) and a per-block counter (release and acquire at device scope;
-// the wait is bounded and sets *err when it gives up); in between it also
+// a phase between barriers.  Every H iterations a block hands the own rows
+// within H of its edges to the blocks that hold them as halo rows (H may
+// exceed R) through a small global buffer (two parities) and a per-block
+// counter (release and acquire at device scope; the wait is bounded and
+// sets *err when it gives up); in between it also
 // updates the halo rows that are still valid (H - 1 - step rows beyond its
 // own on each side), so only the halo rows move through memory: the state
 // is read once at the start and written once at the end (to f1 when niter
@@ -642,6 +643,7 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
     __syncthreads();
 
     const size_t xrow = (size_t)kLdsState * sx;               // one published row
+    const size_t xblk = (size_t)(ny + 1 - (nb - 1) * R) * xrow;  // one block's rows (the last block's count)
     for (int it = 0; it < niter; ++it) {
         // step st of an exchange period of H iterations: the rows still valid
         // shrink by one per step on each side (g rows beyond the own ones)
@@ -652,10 +654,13 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
         const int ya = max(r0 - g, 1), yz = min(r1 - 1 + g, ny - 1);       // Vy face rows updated
         const int ba = max(r0 - g, 0), bz = min(r1 - 1 + g, ny);           // rows of the free-slip copies
         if (it > 0 && st == 0) {
-            // the neighbours' rows of the previous iteration
-            if (tid == 0) {
-                for (int nbr = b - 1; nbr <= b + 1; nbr += 2) {
-                    if (nbr < 0 || nbr >= nb) continue;
+            // the halo rows of the previous iteration from the blocks that own
+            // them (H may exceed R: then from more than one block on a side);
+            // the lanes of wave 0 wait on one owner each
+            const int hlo = min(ja / R, nb - 1), hhi = min(jz / R, nb - 1);
+            if (tid < 64) {
+                const int nbr = hlo + tid;
+                if (nbr <= hhi && nbr != b) {
                     for (int spins = 0; __hip_atomic_load(flags + nbr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
                                         gen0 + (unsigned long long)(it / H);) {
                         __builtin_amdgcn_s_sleep(1);
@@ -668,13 +673,12 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             __syncthreads();
-            const double* xp = xb + (size_t)((it / H - 1) & 1) * nb * 2 * H * xrow;
-            for (int e = tid; e < 2 * H * W; e += nt) {
-                const int side = e / (H * W), h = (e / W) % H, i = e % W;
-                const int nbr = side == 0 ? b - 1 : b + 1, J = side == 0 ? L0 + h : r1 + h;
-                if (nbr < 0 || nbr >= nb || J > ny) continue;
-                // the upper neighbour's last H own rows, the lower neighbour's first H
-                const double* src = xp + (((size_t)nbr * 2 + (side == 0 ? 1 : 0)) * H + h) * xrow + i;
+            const double* xp = xb + (size_t)((it / H - 1) & 1) * nb * xblk;
+            const int nlo = r0 - ja, nhalo = nlo + (jz - r1 + 1);  // rows ja .. r0-1 and r1 .. jz
+            for (int e = tid; e < nhalo * W; e += nt) {
+                const int k = e / W, i = e % W;
+                const int J = k < nlo ? ja + k : r1 + (k - nlo), o = min(J / R, nb - 1);
+                const double* src = xp + (size_t)o * xblk + (size_t)(J - o * R) * xrow + i;
                 const int l = i + sx * (J - L0);
 #pragma unroll
                 for (int q = 0; q < kLdsState; ++q) smem[q * slot + l] = src[(size_t)q * sx];
@@ -769,12 +773,15 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
         }
         __syncthreads();
         if (st == H - 1 && it + 1 < niter) {
-            // hand the first and last H own rows to the neighbours
-            double* xp = xb + (size_t)((it / H) & 1) * nb * 2 * H * xrow + (size_t)b * 2 * H * xrow;
-            for (int e = tid; e < 2 * H * W; e += nt) {
-                const int side = e / (H * W), h = (e / W) % H, i = e % W;
-                const int l = i + sx * ((side == 0 ? r0 + h : r1 - H + h) - L0);
-                double* dst = xp + ((size_t)side * H + h) * xrow + i;
+            // publish the own rows that other blocks hold as halo rows (those
+            // within H of either edge), at their place in the block's rows
+            double* xp = xb + (size_t)((it / H) & 1) * nb * xblk + (size_t)b * xblk;
+            const int nown = r1 - r0;
+            for (int e = tid; e < nown * W; e += nt) {
+                const int h = e / W, i = e % W;
+                if (h >= H && h < nown - H) continue;
+                const int l = i + sx * (r0 + h - L0);
+                double* dst = xp + (size_t)h * xrow + i;
 #pragma unroll
                 for (int q = 0; q < kLdsState; ++q) dst[(size_t)q * sx] = smem[q * slot + l];
             }
@@ -917,7 +924,7 @@ static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, cons
                 : H == 3 ? k2_lds<VE, MUL, BF, DP, 3>
                 : H == 2 ? k2_lds<VE, MUL, BF, DP, 2>
                          : k2_lds<VE, MUL, BF, DP, 1>;
-    if (nb < 1 || R < H || f0.ny + 1 - (nb - 1) * R < R) return hipErrorInvalidValue;
+    if (nb < 1 || R < 2 || f0.ny + 1 - (nb - 1) * R < R) return hipErrorInvalidValue;
     const size_t bytes = lds2_bytes(f0.sx, f0.ny + 1 - (nb - 1) * R, H, VE, BF);
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
