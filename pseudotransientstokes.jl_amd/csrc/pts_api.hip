@@ -53,6 +53,7 @@ struct pts_ctx {
     int64_t multi_max_cells = (int64_t)1 << 20;
     bool lds_2d = true;                 // 2D small grids: state in LDS for a check interval (k2_lds)
     int lds_rows_2d = 0;                // rows per block of that launch (0: the default choice)
+    int lds_cols_2d = 0;                // columns per block of that launch (0: the whole width)
     int lds_threads_2d = 1024;          // threads per block of that launch
     int lds_halo_2d = 0;                // halo rows (1 to 4; 0: the default choice): rows exchanged every that many iterations
                                         // (127^2: 5.4 us per iteration with 2, 6.2 with 1)
@@ -380,6 +381,7 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_2D_MULTI")) c->multi_2d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_2D_LDS")) c->lds_2d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_2D_LDS_ROWS")) c->lds_rows_2d = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("PTS_2D_LDS_COLS")) c->lds_cols_2d = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("PTS_2D_LDS_THREADS")) c->lds_threads_2d = std::min(1024, std::max(64, std::atoi(e) / 64 * 64));
     if (const char* e = std::getenv("PTS_2D_LDS_HALO")) c->lds_halo_2d = std::min(4, std::max(0, std::atoi(e)));
     if (const char* e = std::getenv("PTS_2D_MULTI_ROWS")) c->multi_rows_2d = std::max(1, std::atoi(e));
@@ -969,12 +971,13 @@ static int run_iters_multi(pts_ctx* c, const StepScalars& s0, int64_t n) {
 }
 
 // 2D small grids: n iterations with the state in LDS in one cooperative
-// launch (k2_lds; 1: done; 0: not applicable).  Blocks of R rows, the last
-// one owning the rows left (R .. 2R-1, or more when it takes over the rows
-// of blocks before it, below): R as asked (or the next fewer that works),
+// launch (k2_lds; 1: done; 0: not applicable).  Tiles of R rows and C
+// columns, the last tile of a row or column of tiles owning what is left
+// (R .. 2R-1, or more when it takes over the tile before it): R as asked (or
+// the next fewer that works), C as asked (default: the whole width), each
 // at least 2, within what the LDS holds (160 KiB per CU) with H halo rows
-// on each side (H may exceed R), and few enough blocks to be resident at
-// once.
+// and columns on each side (H may exceed R or C), and few enough blocks to
+// be resident at once.
 static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     if (c->nd != 2 || !c->lds_2d || !single_pass(c) || n < 2 || n > (1 << 30)) return 0;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -985,44 +988,59 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     const bool ve = c->g.ve != 0, mul = c->g.fd_mode == PTS_FD_MUL, bf = c->g.body_force != 0;
     const bool dp = use_dp(c);
     const size_t lds_max = 160 * 1024;
-    const size_t per_row = (size_t)lds2_fields(ve, bf) * c->sx * sizeof(double);  // a row of every held field
-    // the rows a block updates in the steps of an exchange period start at
-    // r0 - (H-1) .. r0: none may be row ny-1 (bc_y!(Vx) copies row ny-2 into
-    // it).  Only the last block's start can be (it is at most ny+1-R): then
-    // the last block takes over the rows of the one before.
-    auto blocks = [&](int r, int h) {
-        int b = std::max(1, (c->ny + 1) / r);
-        while (b > 1 && (b - 1) * r >= c->ny - 1 && (b - 1) * r <= c->ny + h - 2) --b;
+    // tiles of r rows over the positions 0..m: the rows a block updates in the
+    // steps of an exchange period start at r0 - (h-1) .. r0, and none may be
+    // m-1 (bc_y!(Vx) copies row ny-2 into row ny-1, bc_x!(Vy) column nx-2
+    // into column nx-1).  Only the last tile's start can be (it is at most
+    // m+1-r): then the last tile takes over the one before.
+    auto tiles = [](int m, int r, int h) {
+        int b = std::max(1, (m + 1) / r);
+        while (b > 1 && (b - 1) * r >= m - 1 && (b - 1) * r <= m + h - 2) --b;
         return b;
     };
-    auto works = [&](int r, int h) {
-        const int fit = (int)(lds_max / per_row) - 2 * h;  // rows a block can own
-        return r >= 2 && r <= c->ny + 1 && c->ny + 1 - (blocks(r, h) - 1) * r <= fit;
+    auto works = [&](int r, int w, int h) {
+        if (r < 2 || r > c->ny + 1 || w < 2 || w > c->nx + 1) return false;
+        const int rl = c->ny + 1 - (tiles(c->ny, r, h) - 1) * r, cl = c->nx + 1 - (tiles(c->nx, w, h) - 1) * w;
+        return lds2_bytes(lds2_extent(c->ny + 1, r, rl, h), lds2_extent(c->nx + 1, w, cl, h), ve, bf) <= lds_max;
     };
-    int R = 0, H = 0;
-    if (c->lds_rows_2d > 0 || c->lds_halo_2d > 0) {  // as asked, or the next fewer rows that work
-        H = c->lds_halo_2d > 0 ? c->lds_halo_2d : 2;
-        for (int r = c->lds_rows_2d > 0 ? c->lds_rows_2d : 4; r >= 2 && !R; --r)
-            if (works(r, H)) R = r;
-    } else {
-        // short blocks, exchanged every few iterations: many blocks in parallel
-        // against the latency of an exchange through memory (127^2: 3 rows
-        // every 3 iterations 4.4 us per iteration, 2 every 3 or 4 4.5, 4
-        // every 4 4.8, 2 every 2 4.9; 255^2: 3 every 2 7.5 us)
-        const int pairs[][2] = {{3, 3}, {3, 2}, {4, 2}, {2, 2}, {5, 2}, {6, 2}, {7, 2}, {8, 2}};
-        for (const auto& p : pairs)
-            if (works(p[0], p[1])) {
-                R = p[0];
-                H = p[1];
-                break;
-            }
-    }
-    if (R < 2) return 0;
-    const int nb = blocks(R, H);
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (nb > cus) return 0;
+    // columns per tile: as asked, else two tiles across grids at least 64
+    // wide (twice the blocks, half the arithmetic each: 127^2 4.75 -> 4.2 us
+    // per iteration, 255^2 8.1 -> 7.5 us), else the whole width
+    std::vector<int> widths;
+    if (c->lds_cols_2d > 0) widths.push_back(std::min(c->lds_cols_2d, c->nx + 1));
+    else if (c->nx + 1 >= 64) widths.push_back((c->nx + 2) / 2);
+    widths.push_back(c->nx + 1);
+    int R = 0, H = 0, C = 0, nby = 0, nbx = 0;
+    for (int w : widths) {
+        int r_ = 0, h_ = 0;
+        if (c->lds_rows_2d > 0 || c->lds_halo_2d > 0) {  // as asked, or the next fewer rows that work
+            h_ = c->lds_halo_2d > 0 ? c->lds_halo_2d : 2;
+            for (int r = c->lds_rows_2d > 0 ? c->lds_rows_2d : 4; r >= 2 && !r_; --r)
+                if (works(r, w, h_)) r_ = r;
+        } else {
+            // short blocks, exchanged every few iterations: many blocks in
+            // parallel against the latency of an exchange through memory
+            // (127^2, whole width: 3 rows every 3 iterations 4.4 us per
+            // iteration, 2 every 3 or 4 4.5, 4 every 4 4.8, 2 every 2 4.9)
+            const int pairs[][2] = {{3, 3}, {3, 2}, {4, 2}, {2, 2}, {5, 2}, {6, 2}, {7, 2}, {8, 2}};
+            for (const auto& p : pairs)
+                if (works(p[0], w, p[1])) {
+                    r_ = p[0];
+                    h_ = p[1];
+                    break;
+                }
+        }
+        if (r_ < 2) continue;
+        const int ty = tiles(c->ny, r_, h_), tx = tiles(c->nx, w, h_);
+        if (ty * tx > cus) continue;
+        R = r_, H = h_, C = w, nby = ty, nbx = tx;
+        break;
+    }
+    if (R < 2) return 0;
+    const int nb = nby * nbx;
     const Fields f0 = F(c);
     Fields f1 = f0;
     int st[16];
@@ -1037,9 +1055,10 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
         if (!f0.p[f]) return 0;
     for (int q = 0; q < ns; ++q)
         if (!f1.p[st[q]]) return 0;
-    // the exchange buffer: two parities of every block's rows (each block
-    // at the stride of the last block's row count)
-    const size_t xb_bytes = (size_t)2 * nb * (c->ny + 1 - (nb - 1) * R) * 6 * c->sx * sizeof(double);
+    // the exchange buffer: two parities of every block's tile (each at the
+    // size of the last tiles' extents)
+    const size_t xb_bytes = (size_t)2 * nb * (c->ny + 1 - (nby - 1) * R) * 6 * (c->nx + 1 - (nbx - 1) * C) *
+                            sizeof(double);
     if (xb_bytes > c->lds_xb_bytes) {
         if (c->d_lds_xb) (void)hipFree(c->d_lds_xb);
         c->d_lds_xb = nullptr;
@@ -1062,7 +1081,8 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
         c->lds_gen = 0;
         c->lds_nb = nb;
     }
-    hipError_t e = launch2_iter_lds(c->stream, f0, f1, with_dp(c, s0), ve, mul, bf, dp, R, nb, H, (int)n, c->d_lds_xb,
+    hipError_t e = launch2_iter_lds(c->stream, f0, f1, with_dp(c, s0), ve, mul, bf, dp, R, nby, C, nbx, H, (int)n,
+                                    c->d_lds_xb,
                                     c->d_lds_flags, c->lds_gen, (int*)(c->d_bar + 2), c->lds_threads_2d);
     if (e == hipErrorNotSupported || e == hipErrorCooperativeLaunchTooLarge || e == hipErrorInvalidValue) {
         (void)hipGetLastError();
@@ -1325,6 +1345,7 @@ int pts_set_knob(pts_ctx* c, const char* name, int64_t value) {
     else if (k == "PTS_2D_MULTI") c->multi_2d = v != 0;
     else if (k == "PTS_2D_LDS") c->lds_2d = v != 0;
     else if (k == "PTS_2D_LDS_ROWS") c->lds_rows_2d = std::max(0, v);
+    else if (k == "PTS_2D_LDS_COLS") c->lds_cols_2d = std::max(0, v);
     else if (k == "PTS_2D_LDS_THREADS") c->lds_threads_2d = std::min(1024, std::max(64, v / 64 * 64));
     else if (k == "PTS_2D_LDS_HALO") c->lds_halo_2d = (int)std::min<int64_t>(4, std::max<int64_t>(0, v));
     else if (k == "PTS_2D_MULTI_ROWS") c->multi_rows_2d = std::max(1, v);

@@ -192,17 +192,24 @@ hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, cons
 // pts_pt_iterate's ping-pong does.  hipErrorNotSupported when the tiles of
 // `rows` rows do not fit one round of resident blocks.
 // Small 2D grids with the state in LDS (k2_lds): niter iterations in one
-// cooperative launch of nb blocks of R >= 2 rows (the last one: the rows
-// left, at least R), exchanging H rows (1 to 4) every H iterations; xb
-// holds 2 * nb * RL * 6 * sx doubles (RL = ny + 1 - (nb - 1) * R, the last
-// block's rows), flags one counter per block (gen0:
-// their value when the launch starts; they end at gen0 + (niter - 1) / H);
-// threads per block: a multiple of 64 up to 1024.  lds2_bytes: the LDS of a
-// block that owns `rows` rows.
+// cooperative launch of nby x nbx blocks, tiles of R >= 2 rows and C >= 2
+// columns of the positions 0..nx x 0..ny (the last tile of a row or column
+// of tiles: what is left, at least R or C), exchanging H positions (1 to 4)
+// every H iterations; xb holds 2 * nby * nbx * RL * 6 * CL doubles (RL =
+// ny + 1 - (nby - 1) * R, CL = nx + 1 - (nbx - 1) * C: the last tiles'
+// extents), flags one counter per block (gen0: their value when the launch
+// starts; they end at gen0 + (niter - 1) / H); threads per block: a multiple
+// of 64 up to 1024.  lds2_extent: the rows (columns) of LDS a block holds,
+// for tiles of t positions (the last tl) over m with h halo positions on
+// each side, clipped to the grid; lds2_bytes: the LDS of nl x nc positions.
+__host__ __device__ inline int lds2_extent(int m, int t, int tl, int h) {
+    const int e = (t > tl ? t : tl) + 2 * h;
+    return e < m ? e : m;
+}
 int lds2_fields(bool ve, bool bf);
-size_t lds2_bytes(int sx, int rows, int H, bool ve, bool bf);
+size_t lds2_bytes(int nl, int nc, bool ve, bool bf);
 hipError_t launch2_iter_lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, bool ve,
-                            bool mul, bool bf, bool dp, int R, int nb, int H, int niter, double* xb,
+                            bool mul, bool bf, bool dp, int R, int nby, int C, int nbx, int H, int niter, double* xb,
                             unsigned long long* flags, unsigned long long gen0, int* err, int threads);
 hipError_t launch2_iter_multi(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
                               bool mul, bool bf, int rows, int niter, unsigned* bar, int* err);

@@ -556,45 +556,53 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
 }
 
 // ---------------------------------------------------------------------------
-// Small 2D grids: the whole state in LDS for a whole check interval.  Block b
-// holds rows [r0, r1) = [b·R, b·R + R) of every field (the last block: from
-// b·R up to row ny, at least R rows), plus H halo rows on each side, and runs
-// `niter` iterations on them: the updates of
-// compute_P!/compute_τ! (cells and vertices, in place) and of compute_V!
-// (faces, in place, with the free-slip copies of bc_y!(Vx), bc_x!(Vy)), each
-// a phase between barriers.  Every H iterations a block hands the own rows
-// within H of its edges to the blocks that hold them as halo rows (H may
-// exceed R) through a small global buffer (two parities) and a per-block
-// counter (release and acquire at device scope; the wait is bounded and
-// sets *err when it gives up); in between it also
-// updates the halo rows that are still valid (H - 1 - step rows beyond its
-// own on each side), so only the halo rows move through memory: the state
-// is read once at the start and written once at the end (to f1 when niter
-// is odd, else to f0; in the divide form the velocities before the last
-// iteration go to the other one, as the ping-pong leaves them for the err).
-// A cooperative launch keeps every block resident.  Bit-identical to the
-// per-kernel sequence.
+// Small 2D grids: the whole state in LDS for a whole check interval.  The
+// positions (i, J), 0 <= i <= nx, 0 <= J <= ny, are dealt in tiles of R rows
+// and C columns (the last tile of a row or column of tiles: what is left,
+// at least R or C); block b = by·nbx + bx holds its tile of every field plus
+// H halo rows and columns on each side, and runs `niter` iterations on them:
+// the updates of compute_P!/compute_τ! (cells and vertices, in place) and of
+// compute_V! (faces, in place, with the free-slip copies of bc_y!(Vx),
+// bc_x!(Vy)), each a phase between barriers.  Every H iterations a block
+// hands its positions within H of its edges to the blocks that hold them as
+// halo (H may exceed R or C) through a small global buffer (two parities)
+// and a per-block counter (release and acquire at device scope; the wait is
+// bounded and sets *err when it gives up); in between it also updates the
+// halo positions that are still valid (H - 1 - step beyond its own on each
+// side), so only the halo moves through memory: the state is read once at
+// the start and written once at the end (to f1 when niter is odd, else to
+// f0; in the divide form the velocities before the last iteration go to the
+// other one, as the ping-pong leaves them for the err).  A cooperative
+// launch keeps every block resident.  Bit-identical to the per-kernel
+// sequence.
 // ---------------------------------------------------------------------------
 constexpr int kLdsState = 6;  // Pt, τxx, τyy, τxy, Vx, Vy
 __device__ __forceinline__ int lds_state_field(int q) {
     return q == 0 ? PTS_PT : q == 1 ? PTS_TXX : q == 2 ? PTS_TYY : q == 3 ? PTS_TXY : q == 4 ? PTS_VX : PTS_VY;
 }
 
-// LDS slot q holds rows L0 .. L0+NL-1 of one field; element (i, J) is at
-// i + sx·(J - L0) of the slot (a local index: every pointer stays inside the
-// LDS array).  Slots 0-5: the state (lds_state_field order), 6 μ, 7 Gdτ,
-// 8 dτ_Rho, then ρg (BF) and τxx_o, τyy_o, τxy_o (VE).
+// LDS slot q holds rows L0 = max(r0 - H, 0) .. and columns K0 = max(c0 - H,
+// 0) .. of one field, lds2_extent of each (the most any tile holds);
+// element (i, J) is at (i - K0) + sl·(J - L0) of the slot, sl the column
+// extent (a local index: every pointer stays inside the LDS array).  Slots
+// 0-5: the state (lds_state_field order), 6 μ, 7 Gdτ, 8 dτ_Rho, then ρg
+// (BF) and τxx_o, τyy_o, τxy_o (VE).
 template <bool VE, bool MUL, bool BF, bool DP, int H>
-__global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1, const StepScalars c, int R,
-                                               int niter, double* xb, unsigned long long* flags,
+__global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1, const StepScalars c, int R, int C,
+                                               int nbx, int niter, double* xb, unsigned long long* flags,
                                                unsigned long long gen0, int* err) {
     extern __shared__ double smem[];
     const int nx = f0.nx, ny = f0.ny, sx = f0.sx;
     const int nb = gridDim.x, b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-    const int r0 = b * R, r1 = b == nb - 1 ? ny + 1 : r0 + R;  // own rows (Vy has faces up to row ny)
-    const int L0 = r0 - H, NL = ny + 1 - (nb - 1) * R + 2 * H;  // LDS rows L0 .. L0 + NL - 1 (the last block's count)
-    const int lb = L0 * sx;                           // global index of local index 0
-    const size_t slot = (size_t)NL * sx;
+    const int nby = nb / nbx, by = b / nbx, bx = b % nbx;
+    // own rows and columns (Vy has faces up to row ny, Vx up to column nx);
+    // the last tile of a row or column of tiles owns what is left
+    const int r0 = by * R, r1 = by == nby - 1 ? ny + 1 : r0 + R;
+    const int c0 = bx * C, c1 = bx == nbx - 1 ? nx + 1 : c0 + C;
+    const int RL = ny + 1 - (nby - 1) * R, CL = nx + 1 - (nbx - 1) * C;  // the last tiles' extents
+    const int L0 = max(r0 - H, 0), K0 = max(c0 - H, 0);  // LDS row and column 0
+    const int sl = lds2_extent(nx + 1, C, CL, H);        // LDS row stride
+    const size_t slot = (size_t)lds2_extent(ny + 1, R, RL, H) * sl;
     double* const Pt = smem;
     double* const txx = smem + slot;
     double* const tyy = smem + 2 * slot;
@@ -615,12 +623,14 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
     lf.p[PTS_TYY] = tyy;
     lf.p[PTS_TXY] = txy;
     lf.p[PTS_RHOG] = rg;
-    const int W = nx + 1;  // columns of the widest field
-    const int ja = max(L0, 0), jz = min(r1 - 1 + H, ny);  // rows held
+    lf.sx = sl;
+    const int ja = L0, jz = min(r1 - 1 + H, ny);  // rows held
+    const int ia = K0, iz = min(c1 - 1 + H, nx);  // columns held
+    const int nw = iz - ia + 1, ncol = c1 - c0;
 
-    // load the tile (rows outside the grid are never read)
-    for (int e = tid; e < (jz - ja + 1) * W; e += nt) {
-        const int J = ja + e / W, i = e % W, o = i + sx * J, l = o - lb;
+    // load the tile (positions outside the grid are never read)
+    for (int e = tid; e < (jz - ja + 1) * nw; e += nt) {
+        const int J = ja + e / nw, i = ia + e % nw, o = i + sx * J, l = (i - K0) + sl * (J - L0);
 #pragma unroll
         for (int q = 0; q < kLdsState; ++q) smem[q * slot + l] = f0.p[lds_state_field(q)][o];
         Mu[l] = f0.p[PTS_MUS][o];
@@ -642,25 +652,32 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
     }
     __syncthreads();
 
-    const size_t xrow = (size_t)kLdsState * sx;               // one published row
-    const size_t xblk = (size_t)(ny + 1 - (nb - 1) * R) * xrow;  // one block's rows (the last block's count)
+    // published positions: per block RL rows of kLdsState rows of CL values
+    const size_t xblk = (size_t)RL * kLdsState * CL;
     for (int it = 0; it < niter; ++it) {
-        // step st of an exchange period of H iterations: the rows still valid
-        // shrink by one per step on each side (g rows beyond the own ones)
+        // step st of an exchange period of H iterations: the rows and columns
+        // still valid shrink by one per step on each side (g beyond the own
+        // ones); columns mirror rows (Vx is to columns what Vy is to rows)
         const int st = it % H, g = H - 1 - st;
         const int ca = max(r0 - H + st, 0), cz = min(r1 - 1 + g, ny - 1);  // cell rows of the P, τ phase
         const int vz = min(r1 - 1 + g, ny - 2);                            // vertex rows ca .. vz
-        const int xa = max(r0 - g, 1), xz = min(r1 - 1 + g, ny - 2);       // Vx face rows updated
-        const int ya = max(r0 - g, 1), yz = min(r1 - 1 + g, ny - 1);       // Vy face rows updated
+        const int xz = min(r1 - 1 + g, ny - 2);                            // Vx face rows ya .. xz
+        const int ya = max(r0 - g, 1), yz = min(r1 - 1 + g, ny - 1);       // Vy face rows
         const int ba = max(r0 - g, 0), bz = min(r1 - 1 + g, ny);           // rows of the free-slip copies
+        const int ka = max(c0 - H + st, 0), kz = min(c1 - 1 + g, nx - 1);  // cell columns
+        const int vkz = min(c1 - 1 + g, nx - 2);                           // vertex columns ka .. vkz
+        const int fa = max(c0 - g, 1), fz = min(c1 - 1 + g, nx - 1);       // Vx face columns
+        const int fyz = min(c1 - 1 + g, nx - 2);                           // Vy face columns fa .. fyz
+        const int qa = max(c0 - g, 0), qz = min(c1 - 1 + g, nx);           // columns of the free-slip copies
         if (it > 0 && st == 0) {
-            // the halo rows of the previous iteration from the blocks that own
-            // them (H may exceed R: then from more than one block on a side);
-            // the lanes of wave 0 wait on one owner each
-            const int hlo = min(ja / R, nb - 1), hhi = min(jz / R, nb - 1);
+            // the halo positions of the previous iteration from the blocks
+            // that own them (H may exceed R or C: then from more than one
+            // block on a side); the lanes of wave 0 wait on one owner each
+            const int tylo = min(ja / R, nby - 1), tyhi = min(jz / R, nby - 1);
+            const int txlo = min(ia / C, nbx - 1), txhi = min(iz / C, nbx - 1), ntx = txhi - txlo + 1;
             if (tid < 64) {
-                const int nbr = hlo + tid;
-                if (nbr <= hhi && nbr != b) {
+                const int k = tid, nbr = (tylo + k / ntx) * nbx + txlo + k % ntx;
+                if (k < (tyhi - tylo + 1) * ntx && nbr != b) {
                     for (int spins = 0; __hip_atomic_load(flags + nbr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
                                         gen0 + (unsigned long long)(it / H);) {
                         __builtin_amdgcn_s_sleep(1);
@@ -673,15 +690,46 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             __syncthreads();
+            // the halo: the held rows below and above the tile (all held
+            // columns), then the held columns left and right of it (the
+            // tile's rows); two positions per thread and round, loads first
             const double* xp = xb + (size_t)((it / H - 1) & 1) * nb * xblk;
-            const int nlo = r0 - ja, nhalo = nlo + (jz - r1 + 1);  // rows ja .. r0-1 and r1 .. jz
-            for (int e = tid; e < nhalo * W; e += nt) {
-                const int k = e / W, i = e % W;
-                const int J = k < nlo ? ja + k : r1 + (k - nlo), o = min(J / R, nb - 1);
-                const double* src = xp + (size_t)o * xblk + (size_t)(J - o * R) * xrow + i;
-                const int l = i + sx * (J - L0);
+            const int nbot = (r0 - ja) * nw, ntop = (jz - r1 + 1) * nw, nside = r1 - r0;
+            const int nleft = nside * (c0 - ia), nh = nbot + ntop + nleft + nside * (iz - c1 + 1);
+            auto halo_pos = [&](int e, int& J, int& i) {
+                if (e < nbot) {
+                    J = ja + e / nw, i = ia + e % nw;
+                } else if ((e -= nbot) < ntop) {
+                    J = r1 + e / nw, i = ia + e % nw;
+                } else if ((e -= ntop) < nleft) {
+                    J = r0 + e / (c0 - ia), i = ia + e % (c0 - ia);
+                } else {
+                    e -= nleft;
+                    J = r0 + e / (iz - c1 + 1), i = c1 + e % (iz - c1 + 1);
+                }
+            };
+            for (int e = tid; e < nh; e += 2 * nt) {
+                double v[2][kLdsState];
+                int l[2];
 #pragma unroll
-                for (int q = 0; q < kLdsState; ++q) smem[q * slot + l] = src[(size_t)q * sx];
+                for (int u = 0; u < 2; ++u) {
+                    l[u] = -1;
+                    if (e + u * nt >= nh) continue;
+                    int J, i;
+                    halo_pos(e + u * nt, J, i);
+                    const int oy = min(J / R, nby - 1), ox = min(i / C, nbx - 1);
+                    const double* src = xp + (size_t)(oy * nbx + ox) * xblk +
+                                        (size_t)(J - oy * R) * kLdsState * CL + (i - ox * C);
+#pragma unroll
+                    for (int q = 0; q < kLdsState; ++q) v[u][q] = src[(size_t)q * CL];
+                    l[u] = (i - K0) + sl * (J - L0);
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+                    if (l[u] >= 0) {
+#pragma unroll
+                        for (int q = 0; q < kLdsState; ++q) smem[q * slot + l[u]] = v[u][q];
+                    }
             }
             __syncthreads();
         }
@@ -689,32 +737,32 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
             // the divide-form err takes ∇V from the velocities before the
             // last update, from the ping-pong partner: leave them there
             const Fields& prev = (niter & 1) ? f0 : f1;
-            for (int e = tid; e < (r1 - r0) * W; e += nt) {
-                const int J = r0 + e / W, i = e % W, o = i + sx * J;
-                prev.p[PTS_VX][o] = Vx[o - lb];
-                prev.p[PTS_VY][o] = Vy[o - lb];
+            for (int e = tid; e < (r1 - r0) * ncol; e += nt) {
+                const int J = r0 + e / ncol, i = c0 + e % ncol, o = i + sx * J, l = (i - K0) + sl * (J - L0);
+                prev.p[PTS_VX][o] = Vx[l];
+                prev.p[PTS_VY][o] = Vy[l];
             }
         }
         // compute_P!, compute_τ! (Stokes2D.jl:33-44, Stokes2D_ve.jl:42-55,
         // Stokes2D_ve_perf.jl:47-53): cells and vertices, in place; a thread
         // updates cell (i, J) and vertex (i, J) together (the vertex rows
-        // ca..vz lie within the cell rows ca..cz), two
-        // independent chains in one pass over the rows
-        const int ncell = max(cz - ca + 1, 0) * nx;
+        // and columns lie within the cell rows and columns), two independent
+        // chains in one pass
+        const int nkc = max(kz - ka + 1, 0), ncell = max(cz - ca + 1, 0) * nkc;
         for (int e = tid; e < ncell; e += nt) {
-            const int J = ca + e / nx, i = e % nx, o = i + sx * (J - L0);
+            const int J = ca + e / nkc, i = ka + e % nkc, o = (i - K0) + sl * (J - L0);
             {
                 const double g = Gd[o], m = Mu[o];
                 if constexpr (MUL) {
-                    const double ex = c._dx * (Vx[o + 1] - Vx[o]), ey = c._dy * (Vy[o + sx] - Vy[o]);
+                    const double ex = c._dx * (Vx[o + 1] - Vx[o]), ey = c._dy * (Vy[o + sl] - Vy[o]);
                     Pt[o] = Pt[o] - c.r * g * (ex + ey);
                     const double txo = VE ? txxo[o] : 0.0, tyo = VE ? tyyo[o] : 0.0;
                     txx[o] = tau_upd<VE>(txx[o], txo, g, m, c.Gdt, ex);
                     tyy[o] = tau_upd<VE>(tyy[o], tyo, g, m, c.Gdt, ey);
                 } else {
-                    const double d = (Vx[o + 1] - Vx[o]) / c.dx + (Vy[o + sx] - Vy[o]) / c.dy;
+                    const double d = (Vx[o + 1] - Vx[o]) / c.dx + (Vy[o + sl] - Vy[o]) / c.dy;
                     Pt[o] = Pt[o] - c.r * g * d;
-                    const double ex = Vx[o + 1] - Vx[o], ey = Vy[o + sx] - Vy[o];
+                    const double ex = Vx[o + 1] - Vx[o], ey = Vy[o + sl] - Vy[o];
                     if constexpr (VE) {
                         txx[o] = tau_upd<true>(txx[o], txxo[o], g, m, c.Gdt, ex / c.dx);
                         tyy[o] = tau_upd<true>(tyy[o], tyyo[o], g, m, c.Gdt, ey / c.dy);
@@ -724,11 +772,11 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
                     }
                 }
             }
-            if (J <= vz && i < nx - 1) {
-                const double ag = (Gd[o] + Gd[o + 1] + Gd[o + sx] + Gd[o + sx + 1]) * 0.25;
-                const double am = (Mu[o] + Mu[o + 1] + Mu[o + sx] + Mu[o + sx + 1]) * 0.25;
-                const double ev = 0.5 * (dfd<MUL>(Vx[o + sx + 1] - Vx[o + 1], c.dy, c._dy) +
-                                         dfd<MUL>(Vy[o + sx + 1] - Vy[o + sx], c.dx, c._dx));
+            if (J <= vz && i <= vkz) {
+                const double ag = (Gd[o] + Gd[o + 1] + Gd[o + sl] + Gd[o + sl + 1]) * 0.25;
+                const double am = (Mu[o] + Mu[o + 1] + Mu[o + sl] + Mu[o + sl + 1]) * 0.25;
+                const double ev = 0.5 * (dfd<MUL>(Vx[o + sl + 1] - Vx[o + 1], c.dy, c._dy) +
+                                         dfd<MUL>(Vy[o + sl + 1] - Vy[o + sl], c.dx, c._dx));
                 const double to = VE ? txyo[o] : 0.0;
                 txy[o] = tau_upd<VE>(txy[o], to, ag, am, c.Gdt, ev);
             }
@@ -737,53 +785,56 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
         // compute_dV!, compute_V! (Stokes2D.jl:46-58, Stokes2D_ve_perf.jl:55-59),
         // in place: the residual at index (i-1, J-1) updates face (i, J); the
         // free-slip copies of bc_y!(Vx), bc_x!(Vy) (Stokes2D_ve.jl:75-85) are
-        // stored with it (this block updates the row it copies: R >= 2, and
-        // the updated rows never start at ny-1), and the copies of the
-        // Dirichlet corners (columns 0, nx of Vx; rows 0, ny of Vy) follow
-        // (a thread updates Vx and Vy of face (i, J) together: the Vx rows
-        // xa..xz lie within the Vy rows ya..yz, xa = ya, xz <= yz)
-        const int nface = max(yz - ya + 1, 0) * (nx - 1);
+        // stored with it (a block updates the row or column it copies: R, C
+        // >= 2, and the updated rows or columns never start at ny-1 or nx-1),
+        // and the copies of the Dirichlet corners (columns 0, nx of Vx; rows
+        // 0, ny of Vy) follow.  A thread updates Vx and Vy of face (i, J)
+        // together: the Vx rows ya..xz lie within the Vy rows ya..yz, the Vy
+        // columns fa..fyz within the Vx columns fa..fz.
+        const int nkf = max(fz - fa + 1, 0), nface = max(yz - ya + 1, 0) * nkf;
         for (int e = tid; e < nface + 8; e += nt) {
             if (e < nface) {
-                const int J = ya + e / (nx - 1), i = 1 + e % (nx - 1), o = (i - 1) + sx * (J - 1 - L0);
+                const int J = ya + e / nkf, i = fa + e % nkf, o = (i - 1 - K0) + sl * (J - 1 - L0);
                 if (J <= xz) {
                     const double Rr = res2_x<MUL>(lf, c, o);
-                    const double v = Vx[o + sx + 1] + Rr * ((Rh[o + sx] + Rh[o + sx + 1]) * 0.5);
-                    Vx[o + sx + 1] = v;
+                    const double v = Vx[o + sl + 1] + Rr * ((Rh[o + sl] + Rh[o + sl + 1]) * 0.5);
+                    Vx[o + sl + 1] = v;
                     if (J == 1) Vx[o + 1] = v;                   // Vx[:, 0] = Vx[:, 1]
-                    if (J == ny - 2) Vx[o + 2 * sx + 1] = v;     // Vx[:, ny-1] = Vx[:, ny-2]
+                    if (J == ny - 2) Vx[o + 2 * sl + 1] = v;     // Vx[:, ny-1] = Vx[:, ny-2]
                 }
-                if (i < nx - 1) {
+                if (i <= fyz) {
                     const double Rr = res2_y<MUL>(lf, c, o, BF);
-                    const double v = Vy[o + sx + 1] + Rr * ((Rh[o + 1] + Rh[o + sx + 1]) * 0.5);
-                    Vy[o + sx + 1] = v;
-                    if (i == 1) Vy[o + sx] = v;                  // Vy[0, :] = Vy[1, :]
-                    if (i == nx - 2) Vy[o + sx + 2] = v;         // Vy[nx-1, :] = Vy[nx-2, :]
+                    const double v = Vy[o + sl + 1] + Rr * ((Rh[o + 1] + Rh[o + sl + 1]) * 0.5);
+                    Vy[o + sl + 1] = v;
+                    if (i == 1) Vy[o + sl] = v;                  // Vy[0, :] = Vy[1, :]
+                    if (i == nx - 2) Vy[o + sl + 2] = v;         // Vy[nx-1, :] = Vy[nx-2, :]
                 }
             } else {
                 const int k = e - nface;
                 if (k < 4) {  // Vx at the Dirichlet columns 0 and nx, rows 0 and ny-1
                     const int i = (k & 1) ? nx : 0, J = k < 2 ? 0 : ny - 1, Js = k < 2 ? 1 : ny - 2;
-                    if (J >= ba && J <= bz) Vx[i + sx * (J - L0)] = Vx[i + sx * (Js - L0)];
+                    if (J >= ba && J <= bz && i >= qa && i <= qz)
+                        Vx[(i - K0) + sl * (J - L0)] = Vx[(i - K0) + sl * (Js - L0)];
                 } else {      // Vy at the Dirichlet rows 0 and ny, columns 0 and nx-1
                     const int J = (k & 1) ? ny : 0, i = k < 6 ? 0 : nx - 1, is = k < 6 ? 1 : nx - 2;
-                    if (J >= ba && J <= bz) Vy[i + sx * (J - L0)] = Vy[is + sx * (J - L0)];
+                    if (J >= ba && J <= bz && i >= qa && i <= qz)
+                        Vy[(i - K0) + sl * (J - L0)] = Vy[(is - K0) + sl * (J - L0)];
                 }
             }
         }
         __syncthreads();
         if (st == H - 1 && it + 1 < niter) {
-            // publish the own rows that other blocks hold as halo rows (those
-            // within H of either edge), at their place in the block's rows
+            // publish the own positions that other blocks hold as halo (those
+            // within H of an edge of the tile), at their place in the tile
             double* xp = xb + (size_t)((it / H) & 1) * nb * xblk + (size_t)b * xblk;
-            const int nown = r1 - r0;
-            for (int e = tid; e < nown * W; e += nt) {
-                const int h = e / W, i = e % W;
-                if (h >= H && h < nown - H) continue;
-                const int l = i + sx * (r0 + h - L0);
-                double* dst = xp + (size_t)h * xrow + i;
+            const int nrow = r1 - r0;
+            for (int e = tid; e < nrow * ncol; e += nt) {
+                const int h = e / ncol, w = e % ncol;
+                if (h >= H && h < nrow - H && w >= H && w < ncol - H) continue;
+                const int l = (c0 + w - K0) + sl * (r0 + h - L0);
+                double* dst = xp + (size_t)h * kLdsState * CL + w;
 #pragma unroll
-                for (int q = 0; q < kLdsState; ++q) dst[(size_t)q * sx] = smem[q * slot + l];
+                for (int q = 0; q < kLdsState; ++q) dst[(size_t)q * CL] = smem[q * slot + l];
             }
             __syncthreads();
             if (tid == 0) {
@@ -793,12 +844,12 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
             }
         }
     }
-    // the own rows of the state to the buffer the iterations end in
+    // the own positions of the state to the buffer the iterations end in
     const Fields& out = (niter & 1) ? f1 : f0;
-    for (int e = tid; e < (r1 - r0) * W; e += nt) {
-        const int J = r0 + e / W, i = e % W, o = i + sx * J;
+    for (int e = tid; e < (r1 - r0) * ncol; e += nt) {
+        const int J = r0 + e / ncol, i = c0 + e % ncol, o = i + sx * J, l = (i - K0) + sl * (J - L0);
 #pragma unroll
-        for (int q = 0; q < kLdsState; ++q) out.p[lds_state_field(q)][o] = smem[q * slot + (o - lb)];
+        for (int q = 0; q < kLdsState; ++q) out.p[lds_state_field(q)][o] = smem[q * slot + l];
     }
 }
 
@@ -910,37 +961,38 @@ static void go2(hipStream_t s, const Fields& in, const Fields& out, const StepSc
     }
 }
 
-// Fields held in LDS per row by k2_lds, and its LDS bytes for R own rows.
+// Fields held in LDS by k2_lds, and its LDS bytes for nl x nc positions.
 int lds2_fields(bool ve, bool bf) { return 9 + (bf ? 1 : 0) + (ve ? 3 : 0); }
-size_t lds2_bytes(int sx, int rows, int H, bool ve, bool bf) {
-    return (size_t)lds2_fields(ve, bf) * (rows + 2 * H) * sx * sizeof(double);
+size_t lds2_bytes(int nl, int nc, bool ve, bool bf) {
+    return (size_t)lds2_fields(ve, bf) * nl * nc * sizeof(double);
 }
 
 template <bool VE, bool MUL, bool BF, bool DP>
-static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, int R, int nb,
-                         int H, int niter, double* xb, unsigned long long* flags, unsigned long long gen0, int* err,
-                         int threads) {
+static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, int R, int nby,
+                         int C, int nbx, int H, int niter, double* xb, unsigned long long* flags,
+                         unsigned long long gen0, int* err, int threads) {
     auto kern = H == 4   ? k2_lds<VE, MUL, BF, DP, 4>
                 : H == 3 ? k2_lds<VE, MUL, BF, DP, 3>
                 : H == 2 ? k2_lds<VE, MUL, BF, DP, 2>
                          : k2_lds<VE, MUL, BF, DP, 1>;
-    if (nb < 1 || R < 2 || f0.ny + 1 - (nb - 1) * R < R) return hipErrorInvalidValue;
-    const size_t bytes = lds2_bytes(f0.sx, f0.ny + 1 - (nb - 1) * R, H, VE, BF);
+    const int RL = f0.ny + 1 - (nby - 1) * R, CL = f0.nx + 1 - (nbx - 1) * C;
+    if (nby < 1 || nbx < 1 || R < 2 || C < 2 || RL < R || CL < C) return hipErrorInvalidValue;
+    const size_t bytes = lds2_bytes(lds2_extent(f0.ny + 1, R, RL, H), lds2_extent(f0.nx + 1, C, CL, H), VE, BF);
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     Fields a = f0, bb = f1;
     StepScalars cc = c;
-    int rr = R, ni = niter;
+    int rr = R, cl = C, nx_ = nbx, ni = niter;
     unsigned long long g0 = gen0;
-    void* args[] = {&a, &bb, &cc, &rr, &ni, &xb, &flags, &g0, &err};
-    return hipLaunchCooperativeKernel((const void*)kern, dim3(nb), dim3(threads), args, (unsigned)bytes, s);
+    void* args[] = {&a, &bb, &cc, &rr, &cl, &nx_, &ni, &xb, &flags, &g0, &err};
+    return hipLaunchCooperativeKernel((const void*)kern, dim3(nby * nbx), dim3(threads), args, (unsigned)bytes, s);
 }
 
 hipError_t launch2_iter_lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, bool ve,
-                            bool mul, bool bf, bool dp, int R, int nb, int H, int niter, double* xb,
+                            bool mul, bool bf, bool dp, int R, int nby, int C, int nbx, int H, int niter, double* xb,
                             unsigned long long* flags, unsigned long long gen0, int* err, int threads) {
 #define PTS_LDS_GO(V, M, B, D) \
-    return go2lds<V, M, B, D>(s, f0, f1, c, R, nb, H, niter, xb, flags, gen0, err, threads)
+    return go2lds<V, M, B, D>(s, f0, f1, c, R, nby, C, nbx, H, niter, xb, flags, gen0, err, threads)
     if (ve) {
         if (mul) { if (bf) { if (dp) PTS_LDS_GO(true, true, true, true); PTS_LDS_GO(true, true, true, false); }
                    if (dp) PTS_LDS_GO(true, true, false, true); PTS_LDS_GO(true, true, false, false); }

@@ -175,14 +175,17 @@ def test_cooperative_chunks_equal_single_launches(preset, kw, nout, rows, monkey
     b.close()
 
 
-@pytest.mark.parametrize("preset,kw,nout,env", [
+LDS_GEOMETRIES = [
     ("Stokes2D", dict(n=127, rhog_in=-1.0, ebg=0.0), 10, {}),
     ("Stokes2D", dict(nx=70, ny=42, rhog_in=-1.0), 9, {"PTS_DERIVED_PARAMS": "0"}),
     ("Stokes2D_ve_perf", dict(nx=90, ny=61), 7, {}),
     ("Stokes2D_ve_perf", dict(nx=33, ny=100, rhog_in=-1.0), 12, {"PTS_DERIVED_PARAMS": "0"}),
     ("Stokes2D_ve", dict(n=63, dt=0.1), 200, {}),
     ("Stokes2D_ve", dict(nx=6, ny=5, rad2=9.0, rhog_in=-1.0), 3, {}),
-])
+]
+
+
+@pytest.mark.parametrize("preset,kw,nout,env", LDS_GEOMETRIES)
 @pytest.mark.parametrize("rows", ["0", "2", "3", "7", "40"])
 @pytest.mark.parametrize("halo", ["0", "1", "2", "3", "4"])
 def test_state_in_lds_chunks_equal_single_launches(preset, kw, nout, env, rows, halo, monkeypatch):
@@ -199,6 +202,41 @@ def test_state_in_lds_chunks_equal_single_launches(preset, kw, nout, env, rows, 
         monkeypatch.setenv(k, v)
     monkeypatch.setenv("PTS_2D_LDS_HALO", halo)
     monkeypatch.setenv("PTS_2D_LDS_ROWS", rows)
+    monkeypatch.setenv("PTS_2D_LDS", "1")
+    a = StokesModel(preset, nout=nout, **kw).setup()
+    monkeypatch.setenv("PTS_2D_LDS", "0")
+    monkeypatch.setenv("PTS_2D_MULTI", "0")
+    b = StokesModel(preset, nout=nout, **kw).setup()
+    if a.p["ve"]:
+        a.ctx.assign_tau()
+        b.ctx.assign_tau()
+    params = a.pt_params()
+    for it_max in (29, 40, 3 * nout + 1):
+        ra = a.ctx.pt_solve(params, 1e-300, nout, it_max, 1, use_graph=True)
+        rb = b.ctx.pt_solve(params, 1e-300, nout, it_max, 1, use_graph=True)
+        assert ra["iters"] == rb["iters"] and ra["err"] == rb["err"], (ra, rb)
+        for f in ("Pt", "Vx", "Vy", "txx", "tyy", "txy"):
+            assert np.array_equal(a.get(f), b.get(f)), (f, it_max)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("preset,kw,nout,env", LDS_GEOMETRIES)
+@pytest.mark.parametrize("rows", ["0", "3"])
+@pytest.mark.parametrize("cols", ["2", "7", "40", "100000"])
+@pytest.mark.parametrize("halo", ["0", "1", "3", "4"])
+def test_state_in_lds_column_tiles_equal_single_launches(preset, kw, nout, env, rows, cols, halo, monkeypatch):
+    # the same with tiles of `cols` columns as well (the last tile of a row
+    # of tiles owns the columns left, and those of the tile before when a
+    # block's updated columns would start at column nx-1; 100000: the whole
+    # width; the test above takes the default, two tiles across grids at
+    # least 64 wide): halo positions from up to eight (or, with a halo
+    # deeper than a tile, more) blocks
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    monkeypatch.setenv("PTS_2D_LDS_HALO", halo)
+    monkeypatch.setenv("PTS_2D_LDS_ROWS", rows)
+    monkeypatch.setenv("PTS_2D_LDS_COLS", cols)
     monkeypatch.setenv("PTS_2D_LDS", "1")
     a = StokesModel(preset, nout=nout, **kw).setup()
     monkeypatch.setenv("PTS_2D_LDS", "0")
