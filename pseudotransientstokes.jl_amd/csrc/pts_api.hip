@@ -1008,7 +1008,8 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     // columns per tile: as asked, else two tiles across grids at least 64
     // wide (twice the blocks, half the arithmetic each: 127^2 4.75 -> 4.2 us
-    // per iteration, 255^2 8.1 -> 7.5 us), else the whole width
+    // per iteration, 255^2 8.1 -> 7.5 us, 3 rows every 3 iterations), else
+    // the whole width
     std::vector<int> widths;
     if (c->lds_cols_2d > 0) widths.push_back(std::min(c->lds_cols_2d, c->nx + 1));
     else if (c->nx + 1 >= 64) widths.push_back((c->nx + 2) / 2);
@@ -1023,9 +1024,11 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
         } else {
             // short blocks, exchanged every few iterations: many blocks in
             // parallel against the latency of an exchange through memory
-            // (127^2, whole width: 3 rows every 3 iterations 4.4 us per
-            // iteration, 2 every 3 or 4 4.5, 4 every 4 4.8, 2 every 2 4.9)
-            const int pairs[][2] = {{3, 3}, {3, 2}, {4, 2}, {2, 2}, {5, 2}, {6, 2}, {7, 2}, {8, 2}};
+            // (two tiles across; 127^2: 4 rows every 4 iterations 3.63 us per
+            // iteration, 5 every 4 3.74, 3 every 4 3.75, 4 every 3 3.77, 3
+            // every 3 3.82, 2 every 4 4.0; 255^2: 4 every 4 5.8, 4 every 3
+            // 6.1, 3 every 3 6.7, 3 every 2 8.0)
+            const int pairs[][2] = {{4, 4}, {3, 3}, {3, 2}, {4, 2}, {2, 2}, {5, 2}, {6, 2}, {7, 2}, {8, 2}};
             for (const auto& p : pairs)
                 if (works(p[0], w, p[1])) {
                     r_ = p[0];
