@@ -63,6 +63,8 @@ struct pts_ctx {
     size_t lds_xb_bytes = 0;            // size of the exchange buffer
     unsigned long long lds_gen = 0;     // the counters' value between launches
     int lds_nb = 0;                     // blocks of the last launch (whose counters all hold lds_gen)
+    bool lds_fault_wait = false;        // test hook: the exchange waits target a generation never reached
+    int lds_wait_spins = 1 << 22;       // polls before an exchange wait gives up
     unsigned* d_bar = nullptr;          // barrier count, generation, error flag
     int* h_bar_err = nullptr;           // pinned copy of the error flag
     int planes_per_block = -160; // 3D single-pass: z planes per block (<0: fewest block rounds, capped)
@@ -382,6 +384,8 @@ int pts_create(const pts_grid* g, pts_ctx** out) {
     if (const char* e = std::getenv("PTS_2D_LDS")) c->lds_2d = std::atoi(e) != 0;
     if (const char* e = std::getenv("PTS_2D_LDS_ROWS")) c->lds_rows_2d = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("PTS_2D_LDS_COLS")) c->lds_cols_2d = std::max(0, std::atoi(e));
+    if (const char* e = std::getenv("PTS_2D_LDS_FAULT_WAIT")) c->lds_fault_wait = std::atoi(e) != 0;
+    if (const char* e = std::getenv("PTS_2D_LDS_WAIT_SPINS")) c->lds_wait_spins = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("PTS_2D_LDS_THREADS")) c->lds_threads_2d = std::min(1024, std::max(64, std::atoi(e) / 64 * 64));
     if (const char* e = std::getenv("PTS_2D_LDS_HALO")) c->lds_halo_2d = std::min(4, std::max(0, std::atoi(e)));
     if (const char* e = std::getenv("PTS_2D_MULTI_ROWS")) c->multi_rows_2d = std::max(1, std::atoi(e));
@@ -1084,9 +1088,12 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
         c->lds_gen = 0;
         c->lds_nb = nb;
     }
+    // (the fault hook of the tests: every wait targets a generation no
+    // block reaches and gives up after its bounded spin; the kernel still
+    // runs to its end, and the solve reports it)
     hipError_t e = launch2_iter_lds(c->stream, f0, f1, with_dp(c, s0), ve, mul, bf, dp, R, nby, C, nbx, H, (int)n,
-                                    c->d_lds_xb,
-                                    c->d_lds_flags, c->lds_gen, (int*)(c->d_bar + 2), c->lds_threads_2d);
+                                    c->d_lds_xb, c->d_lds_flags, c->lds_gen, (int*)(c->d_bar + 2), c->lds_threads_2d,
+                                    c->lds_wait_spins, c->lds_fault_wait ? (1ull << 40) : 0ull);
     if (e == hipErrorNotSupported || e == hipErrorCooperativeLaunchTooLarge || e == hipErrorInvalidValue) {
         (void)hipGetLastError();
         return 0;
@@ -1094,6 +1101,7 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
     if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("cooperative launch: ") + hipGetErrorString(e));
     // every block publishes (n - 1) / H times
     c->lds_gen += (unsigned long long)((n - 1) / H);
+    if (c->lds_fault_wait) c->lds_nb = -1;  // the counters start again at zero next time
     e = hipMemcpyAsync(c->h_bar_err, c->d_bar + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e != hipSuccess) return fail(c, PTS_ERR_HIP, std::string("exchange flag: ") + hipGetErrorString(e));
     for (int64_t q = 0; q < n; ++q) swap_state(c);
@@ -1102,7 +1110,8 @@ static int run_iters_lds(pts_ctx* c, const StepScalars& s0, int64_t n) {
 
 // After a synchronisation: a grid barrier of a cooperative launch that gave up.
 static int barrier_error(pts_ctx* c) {
-    if (c->h_bar_err && *c->h_bar_err) return fail(c, PTS_ERR_HIP, "grid barrier wait gave up (results invalid)");
+    if (c->h_bar_err && *c->h_bar_err)
+        return fail(c, PTS_ERR_HIP, "a bounded wait of a cooperative launch gave up (results invalid)");
     return PTS_OK;
 }
 

@@ -199,7 +199,9 @@ hipError_t launch2_iter(hipStream_t s, const Fields& in, const Fields& out, cons
 // ny + 1 - (nby - 1) * R, CL = nx + 1 - (nbx - 1) * C: the last tiles'
 // extents), flags one counter per block (gen0: their value when the launch
 // starts; they end at gen0 + (niter - 1) / H); threads per block: a multiple
-// of 64 up to 1024.  lds2_extent: the rows (columns) of LDS a block holds,
+// of 64 up to 1024; a wait gives up (setting *err) after
+// spin_limit polls, and waits for its generation + wskew (0; the tests raise
+// it so that every wait gives up).  lds2_extent: the rows (columns) of LDS a block holds,
 // for tiles of t positions (the last tl) over m with h halo positions on
 // each side, clipped to the grid; lds2_bytes: the LDS of nl x nc positions.
 __host__ __device__ inline int lds2_extent(int m, int t, int tl, int h) {
@@ -210,7 +212,8 @@ int lds2_fields(bool ve, bool bf);
 size_t lds2_bytes(int nl, int nc, bool ve, bool bf);
 hipError_t launch2_iter_lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, bool ve,
                             bool mul, bool bf, bool dp, int R, int nby, int C, int nbx, int H, int niter, double* xb,
-                            unsigned long long* flags, unsigned long long gen0, int* err, int threads);
+                            unsigned long long* flags, unsigned long long gen0, int* err, int threads, int spin_limit,
+                            unsigned long long wskew);
 hipError_t launch2_iter_multi(hipStream_t s, const Fields& in, const Fields& out, const StepScalars& c, bool ve,
                               bool mul, bool bf, int rows, int niter, unsigned* bar, int* err);
 // 1 in *bad unless Musτ equals compute_maxloc! of μ followed by bc_x!, bc_y!

@@ -567,7 +567,8 @@ __global__ __launch_bounds__(64 * NW, (!PF && NW == 8) ? 6 : 1) void k2_iter(con
 // hands its positions within H of its edges to the blocks that hold them as
 // halo (H may exceed R or C) through a small global buffer (two parities)
 // and a per-block counter (release and acquire at device scope; the wait is
-// bounded and sets *err when it gives up); in between it also updates the
+// bounded by spin_limit polls and sets *err when it gives up; wskew, a test
+// hook, raises the generation waited for); in between it also updates the
 // halo positions that are still valid (H - 1 - step beyond its own on each
 // side), so only the halo moves through memory: the state is read once at
 // the start and written once at the end (to f1 when niter is odd, else to
@@ -639,7 +640,8 @@ __device__ __forceinline__ int lds_state_field(int q) {
 template <bool VE, bool MUL, bool BF, bool DP, int H>
 __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1, const StepScalars c, int R, int C,
                                                int nbx, int niter, double* xb, unsigned long long* flags,
-                                               unsigned long long gen0, int* err) {
+                                               unsigned long long gen0, int* err, int spin_limit,
+                                               unsigned long long wskew) {
     extern __shared__ double smem[];
     const int nx = f0.nx, ny = f0.ny, sx = f0.sx;
     const int nb = gridDim.x, b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
@@ -729,9 +731,9 @@ __global__ __launch_bounds__(1024) void k2_lds(const Fields f0, const Fields f1,
                 const int k = tid, nbr = (tylo + k / ntx) * nbx + txlo + k % ntx;
                 if (k < (tyhi - tylo + 1) * ntx && nbr != b) {
                     for (int spins = 0; __hip_atomic_load(flags + nbr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
-                                        gen0 + (unsigned long long)(it / H);) {
+                                        gen0 + (unsigned long long)(it / H) + wskew;) {
                         __builtin_amdgcn_s_sleep(1);
-                        if (++spins > (1 << 22)) {
+                        if (++spins > spin_limit) {
                             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
                         }
@@ -1020,7 +1022,7 @@ size_t lds2_bytes(int nl, int nc, bool ve, bool bf) {
 template <bool VE, bool MUL, bool BF, bool DP>
 static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, int R, int nby,
                          int C, int nbx, int H, int niter, double* xb, unsigned long long* flags,
-                         unsigned long long gen0, int* err, int threads) {
+                         unsigned long long gen0, int* err, int threads, int spin_limit, unsigned long long wskew) {
     auto kern = H == 4   ? k2_lds<VE, MUL, BF, DP, 4>
                 : H == 3 ? k2_lds<VE, MUL, BF, DP, 3>
                 : H == 2 ? k2_lds<VE, MUL, BF, DP, 2>
@@ -1032,17 +1034,18 @@ static hipError_t go2lds(hipStream_t s, const Fields& f0, const Fields& f1, cons
     if (e != hipSuccess) return e;
     Fields a = f0, bb = f1;
     StepScalars cc = c;
-    int rr = R, cl = C, nx_ = nbx, ni = niter;
-    unsigned long long g0 = gen0;
-    void* args[] = {&a, &bb, &cc, &rr, &cl, &nx_, &ni, &xb, &flags, &g0, &err};
+    int rr = R, cl = C, nx_ = nbx, ni = niter, sp = spin_limit;
+    unsigned long long g0 = gen0, ws = wskew;
+    void* args[] = {&a, &bb, &cc, &rr, &cl, &nx_, &ni, &xb, &flags, &g0, &err, &sp, &ws};
     return hipLaunchCooperativeKernel((const void*)kern, dim3(nby * nbx), dim3(threads), args, (unsigned)bytes, s);
 }
 
 hipError_t launch2_iter_lds(hipStream_t s, const Fields& f0, const Fields& f1, const StepScalars& c, bool ve,
                             bool mul, bool bf, bool dp, int R, int nby, int C, int nbx, int H, int niter, double* xb,
-                            unsigned long long* flags, unsigned long long gen0, int* err, int threads) {
+                            unsigned long long* flags, unsigned long long gen0, int* err, int threads, int spin_limit,
+                            unsigned long long wskew) {
 #define PTS_LDS_GO(V, M, B, D) \
-    return go2lds<V, M, B, D>(s, f0, f1, c, R, nby, C, nbx, H, niter, xb, flags, gen0, err, threads)
+    return go2lds<V, M, B, D>(s, f0, f1, c, R, nby, C, nbx, H, niter, xb, flags, gen0, err, threads, spin_limit, wskew)
     if (ve) {
         if (mul) { if (bf) { if (dp) PTS_LDS_GO(true, true, true, true); PTS_LDS_GO(true, true, true, false); }
                    if (dp) PTS_LDS_GO(true, true, false, true); PTS_LDS_GO(true, true, false, false); }

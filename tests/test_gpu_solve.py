@@ -276,3 +276,31 @@ def test_state_in_lds_block_count_changes_between_launches(monkeypatch):
             assert np.array_equal(a.get(f), b.get(f)), (f, rows, halo)
     a.close()
     b.close()
+
+
+def test_state_in_lds_wait_that_gives_up_is_reported(monkeypatch):
+    # failure handling of the exchange: with the test hook
+    # PTS_2D_LDS_FAULT_WAIT=1 every wait targets a generation that no block
+    # reaches, so each ends at its bounded spin (1000 polls here); the kernels
+    # run to their end and the solve reports the error instead of returning
+    # the invalid state
+    monkeypatch.setenv("PTS_2D_LDS", "1")
+    monkeypatch.setenv("PTS_2D_LDS_FAULT_WAIT", "1")
+    monkeypatch.setenv("PTS_2D_LDS_WAIT_SPINS", "1000")
+    m = StokesModel("Stokes2D", n=63, nout=10).setup()
+    with pytest.raises(PtsError) as e:
+        m.ctx.pt_solve(m.pt_params(), 1e-300, 10, 29, 1, use_graph=True)
+    assert "gave up" in str(e.value), str(e.value)
+    m.close()
+    # a context without the hook runs the same solve as one launch per iteration
+    monkeypatch.setenv("PTS_2D_LDS_FAULT_WAIT", "0")
+    a = StokesModel("Stokes2D", n=63, nout=10).setup()
+    monkeypatch.setenv("PTS_2D_LDS", "0")
+    monkeypatch.setenv("PTS_2D_MULTI", "0")
+    b = StokesModel("Stokes2D", n=63, nout=10).setup()
+    ra = a.ctx.pt_solve(a.pt_params(), 1e-300, 10, 29, 1, use_graph=True)
+    rb = b.ctx.pt_solve(b.pt_params(), 1e-300, 10, 29, 1, use_graph=True)
+    assert ra["iters"] == rb["iters"] and ra["err"] == rb["err"]
+    assert np.array_equal(a.get("Vx"), b.get("Vx"))
+    a.close()
+    b.close()
